@@ -1,0 +1,672 @@
+// Collision detection of one environment (one lane per environment).
+// Restates genesis/engine/solvers/rigid/collider/{broadphase,narrowphase,mpr,support_field,contact,box_contact,
+// capsule_contact,utils}.py: warm-started sweep-and-prune, then the monolithic convex narrowphase (MPR with cached
+// normal, 4 perturbed re-detections for multi-contact), then the plane-box specialisation. Contacts are appended in the
+// broadphase pair order, which makes the contact list deterministic.
+// Not built: the GJK/EPA fallback of the MPR path (narrowphase.py:726-734); the MPR result is kept instead.
+#pragma once
+#include "rs_device.h"
+
+namespace rs {
+
+constexpr float CCD_EPS = 1e-9f;  // mpr.py:17 (FP32)
+constexpr float CCD_TOL = 1e-6f;
+constexpr int CCD_ITERS = 50;
+
+__device__ inline bool aabb_overlap(const DState& st, int b, int ga, int gb) {
+    for (int k = 0; k < 3; k++) {
+        if (SF(aabb_max, 3 * ga + k) <= SF(aabb_min, 3 * gb + k)) return false;
+        if (SF(aabb_min, 3 * ga + k) >= SF(aabb_max, 3 * gb + k)) return false;
+    }
+    return true;
+}
+
+// broadphase.py:74-138 + :141-396 _func_broad_phase_sap
+__device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) {
+    int nc = SF(n_contacts, 0);
+    for (int i = 0; i < nc; i++) {
+        SF(contact_link_a, i) = SF(contact_link_b, i) = SF(contact_geom_a, i) = SF(contact_geom_b, i) = -1;
+        SF(contact_penetration, i) = 0.f;
+        ST3(contact_pos, i, zero3());
+        ST3(contact_normal, i, zero3());
+        ST3(contact_force, i, zero3());
+    }
+    SF(n_contacts, 0) = 0;
+    const int G = m.G;
+    if (SF(first_time, 0)) {
+        int ib = 0;
+        for (int l = 0; l < m.L; l++)
+            for (int g = m.link_geom_start[l]; g < m.link_geom_end[l]; g++) {
+                SF(sort_value, 2 * ib) = SF(aabb_min, 3 * g);
+                SF(sort_i_g, 2 * ib) = g;
+                SF(sort_is_max, 2 * ib) = 0;
+                SF(sort_value, 2 * ib + 1) = SF(aabb_max, 3 * g);
+                SF(sort_i_g, 2 * ib + 1) = g;
+                SF(sort_is_max, 2 * ib + 1) = 1;
+                ib++;
+            }
+        SF(first_time, 0) = 0;
+    } else {
+        for (int i = 0; i < 2 * G; i++) {
+            int g = SF(sort_i_g, i);
+            SF(sort_value, i) = SF(sort_is_max, i) ? SF(aabb_max, 3 * g) : SF(aabb_min, 3 * g);
+        }
+    }
+    for (int i = 1; i < 2 * G; i++) {
+        float kv = SF(sort_value, i);
+        int km = SF(sort_is_max, i), kg = SF(sort_i_g, i);
+        int j = i - 1;
+        while (j >= 0 && kv < SF(sort_value, j)) {
+            SF(sort_value, j + 1) = SF(sort_value, j);
+            SF(sort_is_max, j + 1) = SF(sort_is_max, j);
+            SF(sort_i_g, j + 1) = SF(sort_i_g, j);
+            j--;
+        }
+        SF(sort_value, j + 1) = kv;
+        SF(sort_is_max, j + 1) = km;
+        SF(sort_i_g, j + 1) = kg;
+    }
+    int n_broad = 0, n_active = 0;
+    for (int i = 0; i < 2 * G; i++) {
+        if (!SF(sort_is_max, i)) {
+            int gi = SF(sort_i_g, i);
+            for (int j = 0; j < n_active; j++) {
+                int ga = SF(active_buffer, j), gb = gi;
+                if (ga > gb) {
+                    int t = ga;
+                    ga = gb;
+                    gb = t;
+                }
+                int pidx = m.collision_pair_idx[ga * G + gb];
+                if (pidx == -1) continue;
+                if (!aabb_overlap(st, b, ga, gb)) {
+                    ST3(cache_normal, pidx, zero3());
+                    continue;
+                }
+                if (n_broad == m.BROAD) {
+                    SF(errbits, 0) = SF(errbits, 0) | RS_ERR_OVERFLOW_CANDIDATE_CONTACTS;
+                    break;
+                }
+                SF(broad_pairs, 2 * n_broad) = ga;
+                SF(broad_pairs, 2 * n_broad + 1) = gb;
+                n_broad++;
+            }
+            SF(active_buffer, n_active) = gi;
+            n_active++;
+        } else {
+            int rem = SF(sort_i_g, i);
+            for (int j = 0; j < n_active; j++)
+                if (SF(active_buffer, j) == rem) {
+                    for (int k = j; k < n_active - 1; k++) SF(active_buffer, k) = SF(active_buffer, k + 1);
+                    n_active--;
+                    break;
+                }
+        }
+    }
+    SF(n_broad, 0) = n_broad;
+}
+
+// ----------------------------------------------------------------------------------------------- support
+
+// support_field.py:139-181: 4-neighbour lookup in the 180x180 (theta, phi) table of a mesh geom
+__device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
+    const int res = 180;
+    const float pi = 3.14159265358979323846f;
+    float theta = atan2f(d.y, d.x);
+    float phi = acosf(clampf(d.z, -1.f, 1.f));
+    float ii = (theta + pi) / pi / 2.f * (float)res;
+    float jj = phi / pi * (float)res;
+    float dmax = -1e20f;
+    f3 v = zero3();
+    const int start = m.geom_support_start[g];
+    for (int i4 = 0; i4 < 4; i4++) {
+        int i = (int)fmodf((i4 & 1) ? ceilf(ii) : floorf(ii), (float)res);
+        int j;
+        if (i4 >> 1) {
+            j = (int)clampf(ceilf(jj), 0.f, (float)(res - 1));
+            if (j == res - 1) j = res - 2;
+        } else {
+            j = (int)clampf(floorf(jj), 0.f, (float)(res - 1));
+            if (j == 0) j = 1;
+        }
+        // NaN directions map to cell (0, 1); keep the index inside the table in every case
+        i = min(max(i, 0), res - 1);
+        const float* pp = &m.support_v[3 * (size_t)(start + i * res + j)];
+        f3 p = mk3(pp[0], pp[1], pp[2]);
+        float dt = dot3(p, d);
+        if (dt > dmax) {
+            v = p;
+            dmax = dt;
+        }
+    }
+    return v;
+}
+
+// mpr.py:147-202 support_driver
+__device__ inline f3 support_driver(const DModel& m, f3 d, int g, f3 pos, f4q quat) {
+    int t = m.geom_type[g];
+    const float* data = &m.geom_data[7 * g];
+    if (t == GT_SPHERE) return pos + d * data[0];
+    if (t == GT_CAPSULE) {
+        f3 axis = qrot(mk3(0.f, 0.f, 1.f), quat);
+        float side = dot3(d, axis) < 0.f ? -1.f : 1.f;
+        f3 ep = pos + (0.5f * data[1]) * side * axis;
+        return ep + d * data[0];
+    }
+    if (t == GT_BOX) {
+        f3 db = qrot_inv(d, quat);
+        f3 vl = mk3((db.x < 0.f ? -1.f : 1.f) * data[0] * 0.5f, (db.y < 0.f ? -1.f : 1.f) * data[1] * 0.5f,
+                    (db.z < 0.f ? -1.f : 1.f) * data[2] * 0.5f);
+        return tr_point(vl, pos, quat);
+    }
+    if (m.geom_support_start[g] < 0) return pos;  // no support table (not a mesh): never read outside the tables
+    f3 dm = qrot(d, qinv(quat));
+    return tr_point(support_mesh(m, dm, g), pos, quat);
+}
+
+// --------------------------------------------------------------------------------------------------- MPR
+struct MPR {
+    f3 v[4], v1[4], v2[4];
+    int size;
+};
+
+__device__ inline void support_pair(const DModel& m, f3 d, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb, f3& v, f3& v1,
+                                    f3& v2) {
+    v1 = support_driver(m, d, ga, pa, qa);
+    v2 = support_driver(m, -d, gb, pb, qb);
+    v = v1 - v2;
+}
+__device__ inline f3 portal_dir(const MPR& s) { return normalize3(cross3(s.v[2] - s.v[1], s.v[3] - s.v[1])); }
+__device__ inline bool reach_tol(const MPR& s, f3 v, f3 dir) {
+    float d1 = dot3(s.v[1], dir), d2 = dot3(s.v[2], dir), d3 = dot3(s.v[3], dir), d4 = dot3(v, dir);
+    float dt = fminf(fminf(d4 - d1, d4 - d2), d4 - d3);
+    return dt < CCD_TOL + CCD_EPS * fmaxf(1.f, dt);
+}
+__device__ inline void expand_portal(MPR& s, f3 v, f3 v1, f3 v2) {
+    f3 v4v0 = cross3(v, s.v[0]);
+    float d = dot3(s.v[1], v4v0);
+    int is;
+    if (d > 0.f) {
+        d = dot3(s.v[2], v4v0);
+        is = d > 0.f ? 1 : 3;
+    } else {
+        d = dot3(s.v[3], v4v0);
+        is = d > 0.f ? 2 : 1;
+    }
+    s.v1[is] = v1;
+    s.v2[is] = v2;
+    s.v[is] = v;
+}
+
+// mpr.py:446-599
+__device__ inline int discover_portal(const DModel& m, MPR& s, int ga, int gb, f3 ca, f3 cb, f3 pa, f4q qa, f3 pb,
+                                      f4q qb) {
+    s.v1[0] = ca;
+    s.v2[0] = cb;
+    s.v[0] = ca - cb;
+    s.size = 1;
+    if (fabsf(s.v[0].x) < CCD_EPS && fabsf(s.v[0].y) < CCD_EPS && fabsf(s.v[0].z) < CCD_EPS) s.v[0].x += 10.f * CCD_EPS;
+    f3 dir = -normalize3(s.v[0]);
+    f3 v, v1, v2;
+    support_pair(m, dir, ga, gb, pa, qa, pb, qb, v, v1, v2);
+    s.v1[1] = v1;
+    s.v2[1] = v2;
+    s.v[1] = v;
+    s.size = 2;
+    if (dot3(v, dir) < CCD_EPS) return -1;
+    dir = cross3(s.v[0], s.v[1]);
+    if (dot3(dir, dir) < CCD_EPS) {
+        if (fabsf(s.v[1].x) < CCD_EPS && fabsf(s.v[1].y) < CCD_EPS && fabsf(s.v[1].z) < CCD_EPS) return 1;
+        return 2;
+    }
+    dir = normalize3(dir);
+    support_pair(m, dir, ga, gb, pa, qa, pb, qb, v, v1, v2);
+    if (dot3(v, dir) < CCD_EPS) return -1;
+    s.v1[2] = v1;
+    s.v2[2] = v2;
+    s.v[2] = v;
+    s.size = 3;
+    dir = normalize3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
+    if (dot3(dir, s.v[0]) > 0.f) {
+        f3 t;
+        t = s.v1[1]; s.v1[1] = s.v1[2]; s.v1[2] = t;
+        t = s.v2[1]; s.v2[1] = s.v2[2]; s.v2[2] = t;
+        t = s.v[1]; s.v[1] = s.v[2]; s.v[2] = t;
+        dir = -dir;
+    }
+    int trials = 0;
+    while (s.size < 4) {
+        support_pair(m, dir, ga, gb, pa, qa, pb, qb, v, v1, v2);
+        if (dot3(v, dir) < CCD_EPS) return -1;
+        bool cont = false;
+        if (dot3(cross3(s.v[1], v), s.v[0]) < -CCD_EPS) {
+            s.v1[2] = v1;
+            s.v2[2] = v2;
+            s.v[2] = v;
+            cont = true;
+        }
+        if (!cont && dot3(cross3(v, s.v[2]), s.v[0]) < -CCD_EPS) {
+            s.v1[1] = v1;
+            s.v2[1] = v2;
+            s.v[1] = v;
+            cont = true;
+        }
+        if (cont) {
+            dir = normalize3(cross3(s.v[1] - s.v[0], s.v[2] - s.v[0]));
+            if (++trials == 15) return -1;
+        } else {
+            s.v1[3] = v1;
+            s.v2[3] = v2;
+            s.v[3] = v;
+            s.size = 4;
+        }
+    }
+    return 0;
+}
+
+// mpr.py:233-276; bounded by the same iteration cap as the penetration search so every lane terminates
+__device__ inline int refine_portal(const DModel& m, MPR& s, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb) {
+    for (int it = 0; it < 1024; it++) {
+        f3 dir = portal_dir(s);
+        if (dot3(s.v[1], dir) > -CCD_EPS) return 0;
+        f3 v, v1, v2;
+        support_pair(m, dir, ga, gb, pa, qa, pb, qb, v, v1, v2);
+        if (!(dot3(v, dir) > -CCD_EPS) || reach_tol(s, v, dir)) return -1;
+        expand_portal(s, v, v1, v2);
+    }
+    return -1;
+}
+
+__device__ inline f3 find_pos(const MPR& s) {
+    float bw[4] = {0.f, 0.f, 0.f, 0.f};
+    float sum = bw[0] + bw[1] + bw[2] + bw[3];
+    if (sum < CCD_EPS) {
+        f3 dir = portal_dir(s);
+        bw[0] = 0.f;
+        bw[1] = dot3(cross3(s.v[2], s.v[3]), dir);
+        bw[2] = dot3(cross3(s.v[3], s.v[1]), dir);
+        bw[3] = dot3(cross3(s.v[1], s.v[2]), dir);
+        sum = bw[0] + bw[1] + bw[2] + bw[3];
+    }
+    f3 p1 = zero3(), p2 = zero3();
+    for (int i = 0; i < 4; i++) {
+        p1 = p1 + bw[i] * s.v1[i];
+        p2 = p2 + bw[i] * s.v2[i];
+    }
+    return (0.5f / sum) * (p1 + p2);
+}
+
+__device__ inline void find_penetration(const DModel& m, MPR& s, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb,
+                                        bool& is_col, f3& n, float& pen, f3& pos) {
+    for (int it = 0;; it++) {
+        f3 dir = portal_dir(s);
+        f3 v, v1, v2;
+        support_pair(m, dir, ga, gb, pa, qa, pb, qb, v, v1, v2);
+        if (reach_tol(s, v, dir) || it > CCD_ITERS) {
+            pen = dot3(dir, s.v[1]);
+            n = -dir;
+            is_col = true;
+            pos = find_pos(s);
+            return;
+        }
+        expand_portal(s, v, v1, v2);
+    }
+}
+
+// mpr.py:602-702 guess_geoms_center
+__device__ inline void guess_centers(const DModel& m, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb, f3 nws, f3& ca,
+                                     f3& cb, float eps) {
+    ca = tr_point(mvec3(m.geom_center, ga), pa, qa);
+    cb = tr_point(mvec3(m.geom_center, gb), pb, qb);
+    if (fabsf(nws.x) > CCD_EPS || fabsf(nws.y) > CCD_EPS || fabsf(nws.z) > CCD_EPS) {
+        const float* A = &m.geom_init_aabb[24 * ga];
+        const float* Bx = &m.geom_init_aabb[24 * gb];
+        f3 a0 = mk3(A[0], A[1], A[2]), a7 = mk3(A[21], A[22], A[23]);
+        f3 b0 = mk3(Bx[0], Bx[1], Bx[2]), b7 = mk3(Bx[21], Bx[22], Bx[23]);
+        ca = tr_point(0.5f * (a7 + a0), pa, qa);
+        cb = tr_point(0.5f * (b7 + b0), pb, qb);
+        f3 delta = ca - cb;
+        f3 nrm = normalize3(delta);
+        if (norm3(cross3(nws, nrm)) > 0.01f) {
+            f3 off = dot3(delta, nws) * nws - delta;
+            float on = norm3(off);
+            if (on > eps) {
+                f3 dir = off / on;
+                f3 dla = qrot_inv(dir, qa), dlb = qrot_inv(dir, qb);
+                f3 sa = a7 - a0, sb = b7 - b0;
+                float la = dot3(sa, mk3(fabsf(dla.x), fabsf(dla.y), fabsf(dla.z)));
+                float lb = dot3(sb, mk3(fabsf(dlb.x), fabsf(dlb.y), fabsf(dlb.z)));
+                float ratio = fminf(on / (la + lb), 0.5f);
+                ca = ca + dir * la * ratio;
+                cb = cb - dir * lb * ratio;
+            }
+        }
+    }
+}
+
+__device__ inline void mpr_contact(const DModel& m, int ga, int gb, f3 nws, f3 pa, f4q qa, f3 pb, f4q qb, bool& is_col,
+                                   f3& n, float& pen, f3& pos) {
+    f3 ca, cb;
+    guess_centers(m, ga, gb, pa, qa, pb, qb, nws, ca, cb, m.eps);
+    MPR s;
+    int res = discover_portal(m, s, ga, gb, ca, cb, pa, qa, pb, qb);
+    is_col = false;
+    n = pos = zero3();
+    pen = 0.f;
+    if (res == 1) {
+        is_col = true;
+        n = -normalize3(s.v[0]);
+        pos = (s.v1[1] + s.v2[1]) * 0.5f;
+    } else if (res == 2) {
+        is_col = true;
+        pen = norm3(s.v[1]);
+        n = -normalize3(s.v[1]);
+        pos = (s.v1[1] + s.v2[1]) * 0.5f;
+    } else if (res == 0) {
+        if (refine_portal(m, s, ga, gb, pa, qa, pb, qb) >= 0) find_penetration(m, s, ga, gb, pa, qa, pb, qb, is_col, n, pen, pos);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------- contacts
+__device__ inline float geom_tolerance(const DModel& m, int ga, int gb, float tol) {
+    const float* B7 = &m.geom_init_aabb[24 * gb];
+    float s = norm3(mk3(B7[21] - B7[0], B7[22] - B7[1], B7[23] - B7[2]));
+    if (m.geom_type[ga] != GT_PLANE) {
+        const float* A = &m.geom_init_aabb[24 * ga];
+        s = fminf(norm3(mk3(A[21] - A[0], A[22] - A[1], A[23] - A[2])), s);
+    }
+    return 0.5f * tol * s;
+}
+
+// contact.py:181-222 func_add_contact
+__device__ inline void add_contact(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3 p, float pen,
+                                   int pair) {
+    int i = SF(n_contacts, 0);
+    if (i < m.KMAX) {
+        float fa = m.geom_friction[ga] * 1.f, fb = m.geom_friction[gb] * 1.f;
+        SF(contact_geom_a, i) = ga;
+        SF(contact_geom_b, i) = gb;
+        ST3(contact_normal, i, n);
+        ST3(contact_pos, i, p);
+        SF(contact_penetration, i) = pen;
+        SF(contact_friction, i) = fmaxf(fmaxf(fa, fb), 1e-2f);
+        for (int k = 0; k < 7; k++)
+            SF(contact_sol_params, 7 * i + k) = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
+        SF(contact_link_a, i) = m.geom_link[ga];
+        SF(contact_link_b, i) = m.geom_link[gb];
+        SF(contact_pair, i) = pair;
+        SF(n_contacts, 0) = i + 1;
+    } else {
+        SF(errbits, 0) = SF(errbits, 0) | RS_ERR_OVERFLOW_COLLISION_PAIRS;
+    }
+}
+
+// contact.py:354-412 (principal inertia axes of the link of the smaller geom)
+__device__ inline void contact_axes(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3& a0, f3& a1) {
+    int g = gb;
+    if (m.geom_type[ga] != GT_PLANE) {
+        const float* A = &m.geom_init_aabb[24 * ga + 21];
+        const float* Bx = &m.geom_init_aabb[24 * gb + 21];
+        g = (A[0] * A[1] * A[2]) < (Bx[0] * Bx[1] * Bx[2]) ? ga : gb;
+    }
+    m33 R = quat_R(LD4(i_quat, m.geom_link[g]), m.eps);
+    int idx = 0;
+    float amax = 0.f;
+    for (int i = 0; i < 3; i++) {
+        float a = fabsf(dot3(mk3(R.a[0][i], R.a[1][i], R.a[2][i]), n));
+        if (a > amax) {
+            amax = a;
+            idx = i;
+        }
+    }
+    idx = (idx + 1) % 3;
+    a0 = mk3(R.a[0][idx], R.a[1][idx], R.a[2][idx]);
+    a0 = normalize3(a0 - dot3(n, a0) * n);
+    a1 = cross3(n, a0);
+}
+
+__device__ inline void rotate_frame(f3 pos, f4q quat, f3 cpos, f4q qr, f3& np, f4q& nq) {
+    nq = qtrans(quat, qr);
+    f3 rel = cpos - pos;
+    np = pos - (qrot(rel, qr) - rel);
+}
+
+__device__ inline f3 qrot_fast(f3 v, f4q q) {
+    f3 u = mk3(q.x, q.y, q.z);
+    f3 t = 2.f * cross3(u, v);
+    return v + q.w * t + cross3(u, t);
+}
+
+// utils.py:9-68
+__device__ inline void closest_segments(f3 a1, f3 a2, f3 b1, f3 b2, float eps, f3& pa, f3& pb) {
+    f3 da = a2 - a1, db = b2 - b1, r = a1 - b1;
+    float aa = dot3(da, da), ab = dot3(da, db), bb = dot3(db, db), d = dot3(da, r), e = dot3(db, r);
+    float den = aa * bb - ab * ab;
+    float s = 0.f, t = 0.f;
+    if (den < eps) {
+        t = bb > eps ? clampf(e / bb, 0.f, 1.f) : 0.f;
+    } else {
+        s = clampf((ab * e - bb * d) / den, 0.f, 1.f);
+        t = clampf(bb > eps ? (ab * s + e) / bb : 0.f, 0.f, 1.f);
+        float sn = clampf(aa > eps ? (ab * t - d) / aa : 0.f, 0.f, 1.f);
+        if (aa > eps) s = sn;
+    }
+    pa = a1 + s * da;
+    pb = b1 + t * db;
+}
+
+// capsule_contact.py:10-99
+__device__ inline bool capsule_capsule(const DModel& m, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb, f3& n, f3& cp,
+                                       float& pen) {
+    float ra = m.geom_data[7 * ga], ha = 0.5f * m.geom_data[7 * ga + 1];
+    float rb = m.geom_data[7 * gb], hb = 0.5f * m.geom_data[7 * gb + 1];
+    f3 z = mk3(0.f, 0.f, 1.f);
+    f3 axa = qrot_fast(z, qa), axb = qrot_fast(z, qb);
+    f3 Pa, Pb;
+    closest_segments(pa - ha * axa, pa + ha * axa, pb - hb * axb, pb + hb * axb, m.eps, Pa, Pb);
+    f3 diff = Pa - Pb;
+    float d2 = dot3(diff, diff), cr = ra + rb;
+    n = mk3(1.f, 0.f, 0.f);
+    cp = zero3();
+    pen = 0.f;
+    if (!(d2 < cr * cr)) return false;
+    float dist = sqrtf(d2);
+    if (dist > m.eps) {
+        n = diff / dist;
+    } else {
+        f3 nd = cross3(axa, axb);
+        float l = dot3(nd, nd);
+        if (l > m.eps) n = nd / l;
+        else n = fabsf(axa.x) < 0.9f ? mk3(0.f, -axa.z, axa.y) : mk3(axa.z, 0.f, -axa.x);
+    }
+    pen = cr - dist;
+    cp = Pa - ra * n + 0.5f * pen * n;
+    return true;
+}
+
+// capsule_contact.py:102-196
+__device__ inline bool sphere_capsule(const DModel& m, int ga, int gb, f3 pa, f4q qa, f3 pb, f4q qb, f3& n, f3& cp,
+                                      float& pen) {
+    float ndir = 1.f;
+    f3 sc = pa, cc = pb;
+    f4q cq = qb;
+    if (m.geom_type[gb] == GT_SPHERE) {
+        int t = ga;
+        ga = gb;
+        gb = t;
+        sc = pb;
+        cc = pa;
+        cq = qa;
+        ndir = -1.f;
+    }
+    float sr = m.geom_data[7 * ga], cr = m.geom_data[7 * gb], ch = 0.5f * m.geom_data[7 * gb + 1];
+    f3 ax = qrot_fast(mk3(0.f, 0.f, 1.f), cq);
+    f3 P1 = cc - ch * ax, P2 = cc + ch * ax, seg = P2 - P1;
+    float sl = dot3(seg, seg), t = 0.5f;
+    if (sl > m.eps) t = clampf(dot3(sc - P1, seg) / sl, 0.f, 1.f);
+    f3 diff = sc - (P1 + t * seg);
+    float d2 = dot3(diff, diff), comb = sr + cr;
+    n = mk3(1.f, 0.f, 0.f);
+    cp = zero3();
+    pen = 0.f;
+    bool col = false;
+    if (d2 < comb * comb) {
+        col = true;
+        float dist = sqrtf(d2);
+        if (dist > m.eps) n = diff / dist;
+        else n = fabsf(ax.x) < 0.9f ? mk3(0.f, -ax.z, ax.y) : mk3(ax.z, 0.f, -ax.x);
+        pen = comb - dist;
+        cp = sc - (sr - 0.5f * pen) * n;
+    }
+    n = n * ndir;
+    return col;
+}
+
+// narrowphase.py:520-970 func_convex_convex_contact (MPR code path)
+__device__ inline void convex_convex(const DModel& m, const DState& st, int b, int ga, int gb) {
+    int ta = m.geom_type[ga], tb = m.geom_type[gb];
+    if (ta == GT_PLANE && tb == GT_BOX) return;
+    const float eps = m.eps, mcp = m.mc_perturbation;
+    bool multi = m.enable_multi_contact && ta != GT_SPHERE && ta != GT_ELLIPSOID && tb != GT_SPHERE && tb != GT_ELLIPSOID;
+    float tolerance = geom_tolerance(m, ga, gb, m.mc_tolerance);
+    f3 pa0 = LD3(geoms_pos, ga), pb0 = LD3(geoms_pos, gb);
+    f4q qa0 = LD4(geoms_quat, ga), qb0 = LD4(geoms_quat, gb);
+    f3 pa = pa0, pb = pb0;
+    f4q qa = qa0, qb = qb0;
+    bool col0 = false, col = false;
+    float pen0 = 0.f, pen = 0.f;
+    f3 n0 = zero3(), p0 = zero3(), n = zero3(), cp = zero3(), ax0 = zero3(), ax1 = zero3();
+    f4q qr{0.f, 0.f, 0.f, 0.f};
+    int n_con = 0;
+    int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
+    for (int det = 0; det < 5; det++) {
+        if (multi && col0) {
+            f3 axis = (float)(2 * (det % 2) - 1) * ax0 + (float)(1 - 2 * ((det / 2) % 2)) * ax1;
+            qr = rotvec_quat(mcp * axis, eps);
+            rotate_frame(pa0, qa0, p0, qr, pa, qa);
+            rotate_frame(pb0, qb0, p0, qinv(qr), pb, qb);
+        }
+        if ((multi && col0) || det == 0) {
+            if (ta == GT_CAPSULE && tb == GT_CAPSULE) {
+                col = capsule_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
+            } else if ((ta == GT_SPHERE && tb == GT_CAPSULE) || (ta == GT_CAPSULE && tb == GT_SPHERE)) {
+                col = sphere_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
+            } else if (ta == GT_PLANE) {
+                f3 pd = qrot(mk3(m.geom_data[7 * ga], m.geom_data[7 * ga + 1], m.geom_data[7 * ga + 2]), qa);
+                n = -normalize3(pd);
+                f3 v1 = support_driver(m, n, gb, pb, qb);
+                pen = dot3(n, v1 - pa);
+                cp = v1 - 0.5f * pen * n;
+                col = pen > 0.f;
+            } else {
+                bool updated = false;
+                f3 nws = LD3(cache_normal, pair);
+                bool avail = fabsf(nws.x) > eps || fabsf(nws.y) > eps || fabsf(nws.z) > eps;
+                for (int i_mpr = 0; i_mpr < 2; i_mpr++) {
+                    if (i_mpr == 1 && det == 0 && !col && avail) {
+                        nws = zero3();
+                        avail = false;
+                        updated = false;
+                    }
+                    if (!updated) {
+                        mpr_contact(m, ga, gb, nws, pa, qa, pb, qb, col, n, pen, cp);
+                        updated = true;
+                    }
+                }
+            }
+        }
+        if (det == 0) {
+            col0 = col;
+            n0 = n;
+            pen0 = pen;
+            p0 = cp;
+            if (col0) {
+                add_contact(m, st, b, ga, gb, n0, p0, pen0, pair);
+                if (multi) {
+                    contact_axes(m, st, b, ga, gb, n, ax0, ax1);
+                    n_con = 1;
+                }
+                ST3(cache_normal, pair, n);
+            } else {
+                ST3(cache_normal, pair, zero3());
+            }
+        } else if (multi && col) {
+            f3 cpa = qrot((cp - 0.5f * pen * n) - p0, qinv(qr)) + p0;
+            f3 cpb = qrot((cp + 0.5f * pen * n) - p0, qr) + p0;
+            cp = 0.5f * (cpa + cpb);
+            f3 tw = cross3(n, n0);
+            tw = mk3(clampf(tw.x, -mcp, mcp), clampf(tw.y, -mcp, mcp), clampf(tw.z, -mcp, mcp));
+            n = n + cross3(tw, n);
+            pen = dot3(n, cpb - cpa);
+            bool rep = false;
+            int nct = SF(n_contacts, 0);
+            for (int i = 0; i < n_con; i++) {
+                int ip = nct - 1 - i;
+                if (!rep && ip >= 0 && norm3(cp - LD3(contact_pos, ip)) < tolerance) rep = true;
+            }
+            if (!rep && pen > -tolerance) {
+                pen = fmaxf(pen, 0.f);
+                add_contact(m, st, b, ga, gb, n, cp, pen, pair);
+                n_con++;
+            }
+        }
+    }
+}
+
+// box_contact.py:25-97 func_plane_box_contact
+__device__ inline void plane_box(const DModel& m, const DState& st, int b, int ga, int gb) {
+    f3 pa = LD3(geoms_pos, ga), pb = LD3(geoms_pos, gb);
+    f4q qa = LD4(geoms_quat, ga), qb = LD4(geoms_quat, gb);
+    int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
+    f3 n = -normalize3(qrot(mk3(m.geom_data[7 * ga], m.geom_data[7 * ga + 1], m.geom_data[7 * ga + 2]), qa));
+    f3 v1 = support_driver(m, n, gb, pb, qb);
+    float pen = dot3(n, v1 - pa);
+    if (!(pen > 0.f)) return;
+    f3 cp = v1 - 0.5f * pen * n;
+    add_contact(m, st, b, ga, gb, n, cp, pen, pair);
+    if (!m.enable_multi_contact) return;
+    int n_con = 1;
+    float tol = geom_tolerance(m, ga, gb, m.mc_tolerance);
+    const float* d = &m.geom_data[7 * gb];
+    for (int c = 0; c < 8; c++) {
+        if (n_con >= m.NCPP) break;
+        f3 loc = mk3(((c & 4) ? 0.5f : -0.5f) * d[0], ((c & 2) ? 0.5f : -0.5f) * d[1], ((c & 1) ? 0.5f : -0.5f) * d[2]);
+        f3 corner = tr_point(loc, pb, qb);
+        float p = dot3(n, corner - pa);
+        if (p > 0.f) {
+            f3 c2 = corner - 0.5f * p * n;
+            if (norm3(c2 - cp) > tol) {
+                add_contact(m, st, b, ga, gb, n, c2, p, pair);
+                n_con++;
+            }
+        }
+    }
+}
+
+// narrowphase.py:1988-2060 + :2135-2189
+__device__ inline void dev_narrowphase(const DModel& m, const DState& st, int b) {
+    int nb = SF(n_broad, 0);
+    for (int i = 0; i < nb; i++) {
+        int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);
+        if (m.geom_type[ga] > m.geom_type[gb]) {
+            int t = ga;
+            ga = gb;
+            gb = t;
+        }
+        int ta = m.geom_type[ga], tb = m.geom_type[gb];
+        if (m.geom_is_convex[ga] && m.geom_is_convex[gb] && tb != GT_TERRAIN &&
+            !(m.box_box_detection && ta == GT_BOX && tb == GT_BOX) && !(ta == GT_PLANE && tb == GT_BOX))
+            convex_convex(m, st, b, ga, gb);
+    }
+    for (int i = 0; i < nb; i++) {
+        int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);
+        if (m.geom_type[ga] > m.geom_type[gb]) {
+            int t = ga;
+            ga = gb;
+            gb = t;
+        }
+        if (m.geom_type[ga] == GT_PLANE && m.geom_type[gb] == GT_BOX) plane_box(m, st, b, ga, gb);
+    }
+}
+
+}  // namespace rs

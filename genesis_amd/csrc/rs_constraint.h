@@ -1,0 +1,561 @@
+// Constraint rows and Newton solver of one environment (one lane per environment, dense rows in [row][env] planes).
+// Restates genesis/engine/solvers/rigid/constraint/solver.py on its batch code path ("monolith" solve body,
+// solver.py:3164-3190) with the exact piecewise-quadratic line search (solver.py:2427-2597).
+#pragma once
+#include "rs_device.h"
+
+namespace rs {
+
+#define JAC(c, d) st.jac[((size_t)(c) * m.D + (d)) * st.B + b]
+
+// solver.py:801-878 func_equality_joint via :881-938
+__device__ inline void dev_equality_constraints(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    int nc = 0, neq = 0;
+    for (int e = 0; e < m.NE; e++) {
+        if (m.eq_type[e] != EQ_JOINT) continue;
+        const float* sp = &m.eq_sol_params[7 * e];
+        const float* data = &m.eq_data[11 * e];
+        int j1 = m.eq_obj1[e], j2 = m.eq_obj2[e];
+        int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
+        int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
+        int c = nc++;
+        neq++;
+        for (int d = 0; d < D; d++) JAC(c, d) = 0.f;
+        float diff = SF(qpos, q2) - m.qpos0[q2];
+        float pos = SF(qpos, q1) - m.qpos0[q1];
+        float deriv = 0.f, dp = 1.f;
+        for (int i5 = 0; i5 < 5; i5++) {
+            pos = pos - dp * data[i5];
+            if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * (float)(i5 + 1);
+            dp = dp * diff;
+        }
+        JAC(c, d1) = 1.f;
+        JAC(c, d2) = -deriv;
+        float jq = JAC(c, d1) * SF(vel, d1) + JAC(c, d2) * SF(vel, d2);
+        float invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+        float imp, aref;
+        imp_aref(sp, -fabsf(pos), jq, pos, imp, aref);
+        float dg = fmaxf(invw * (1.f - imp) / imp, m.eps);
+        SF(diag, c) = dg;
+        SF(aref, c) = aref;
+        SF(efc_D, c) = 1.f / dg;
+    }
+    SF(n_con, 0) = nc;
+    SF(n_con_eq, 0) = neq;
+}
+
+// solver.py:1237-1286, :580-685, :1177-1233
+__device__ inline void dev_inequality_constraints(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    int nc = SF(n_con, 0);
+    int nfl = 0;
+    for (int l = 0; l < m.L; l++)
+        for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++)
+            for (int d = m.joint_dof_start[j]; d < m.joint_dof_end[j]; d++) {
+                if (!(m.dof_frictionloss[d] > m.eps)) continue;
+                float imp, aref;
+                imp_aref(&m.joint_sol_params[7 * j], 0.f, 1.f * SF(vel, d), 0.f, imp, aref);
+                float dg = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+                int c = nc++;
+                nfl++;
+                SF(diag, c) = dg;
+                SF(aref, c) = aref;
+                SF(efc_D, c) = 1.f / dg;
+                SF(efc_frictionloss, c) = m.dof_frictionloss[d];
+                for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
+                JAC(c, d) = 1.f;
+            }
+    SF(n_con_fl, 0) = nfl;
+    if (m.enable_collision) {
+        int ncont = SF(n_contacts, 0);
+        int start = nc;
+        for (int ic = 0; ic < ncont; ic++) {
+            int la = SF(contact_link_a, ic), lb = SF(contact_link_b, ic);
+            f3 cpos = LD3(contact_pos, ic), nrm = LD3(contact_normal, ic);
+            float fr = SF(contact_friction, ic), pen = SF(contact_penetration, ic);
+            float sp[7];
+            for (int k = 0; k < 7; k++) sp[k] = SF(contact_sol_params, 7 * ic + k);
+            f3 d1, d2;
+            orthogonals(nrm, d1, d2);
+            float invw = m.link_invweight[2 * la];
+            if (lb > -1) invw = invw + m.link_invweight[2 * lb];
+            for (int i = 0; i < 4; i++) {
+                f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
+                f3 n = dd * fr - nrm;
+                int c = start + ic * 4 + i;
+                for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
+                float jq = 0.f;
+                for (int ab = 0; ab < 2; ab++) {
+                    float sign = ab == 0 ? -1.f : 1.f;
+                    int link = ab == 0 ? la : lb;
+                    while (link > -1) {
+                        f3 tpos = cpos - LD3(root_COM, link);
+                        for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
+                            f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
+                            f3 vel = qrot(cv - cross3(tpos, ca), qident());
+                            float jj = dot3(sign * vel, n);
+                            jq = jq + jj * SF(vel, k);
+                            JAC(c, k) = JAC(c, k) + jj;
+                        }
+                        link = m.link_parent[link];
+                    }
+                }
+                float imp, aref;
+                imp_aref(sp, -pen, jq, -pen, imp, aref);
+                float dg = invw + fr * fr * invw;
+                dg *= 2.f * fr * fr * (1.f - imp) / imp;
+                dg = fmaxf(dg, m.eps);
+                SF(diag, c) = dg;
+                SF(aref, c) = aref;
+                SF(efc_D, c) = 1.f / dg;
+            }
+        }
+        nc = nc + ncont * 4;
+    }
+    if (m.enable_joint_limit) {
+        for (int l = 0; l < m.L; l++)
+            for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
+                int jt = m.joint_type[j];
+                if (jt != JT_REVOLUTE && jt != JT_PRISMATIC) continue;
+                int q = m.joint_q_start[j], d = m.joint_dof_start[j];
+                float dmin = SF(qpos, q) - m.dof_limit[2 * d];
+                float dmax = m.dof_limit[2 * d + 1] - SF(qpos, q);
+                float pd = fminf(dmin, dmax);
+                if (pd < 0.f) {
+                    float jj = (float)((dmin < dmax) ? 1 : 0) * 2.f - 1.f;
+                    float imp, aref;
+                    imp_aref(&m.joint_sol_params[7 * j], pd, jj * SF(vel, d), pd, imp, aref);
+                    float dg = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+                    int c = nc++;
+                    SF(diag, c) = dg;
+                    SF(aref, c) = aref;
+                    SF(efc_D, c) = 1.f / dg;
+                    for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
+                    JAC(c, d) = jj;
+                }
+            }
+    }
+    SF(n_con, 0) = nc;
+}
+
+// solver.py:2620-2694
+__device__ inline float dev_update_constraint(const DModel& m, const DState& st, int b, float cost_in) {
+    const int D = m.D;
+    int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
+    SF(prev_cost, 0) = cost_in;
+    float cost = 0.f, gauss = 0.f;
+    for (int c = 0; c < nc; c++) {
+        SF(prev_active, c) = SF(active, c);
+        int act = 1;
+        float ff = 0.f;
+        float ja = SF(Jaref, c);
+        if (ne <= c && c < nef) {
+            float f = SF(efc_frictionloss, c), r = SF(diag, c);
+            float rf = r * f;
+            bool ln = ja <= -rf, lp = ja >= rf;
+            act = !(ln || lp);
+            ff = (float)ln * f + (float)lp * -f;
+            float fc = (float)ln * f * (-0.5f * rf - ja);
+            fc = fc + (float)lp * f * (-0.5f * rf + ja);
+            cost = cost + fc;
+        } else if (nef <= c) {
+            act = ja < 0.f;
+        }
+        SF(active, c) = act;
+        SF(efc_force, c) = ff + (-ja * SF(efc_D, c) * (float)act);
+    }
+    for (int d = 0; d < D; d++) {
+        float q = 0.f;
+        for (int c = 0; c < nc; c++) q = q + JAC(c, d) * SF(efc_force, c);
+        SF(qfrc_constraint, d) = q;
+    }
+    for (int d = 0; d < D; d++) {
+        float v = 0.5f * (SF(Ma, d) - SF(force, d)) * (SF(qacc, d) - SF(acc_smooth, d));
+        gauss = gauss + v;
+        cost = cost + v;
+    }
+    for (int c = 0; c < nc; c++) {
+        float ja = SF(Jaref, c);
+        cost = cost + 0.5f * (ja * ja * SF(efc_D, c) * (float)SF(active, c));
+    }
+    SF(gauss, 0) = gauss;
+    return cost;
+}
+
+// solver.py:1395-1452 + :1595-1621
+__device__ inline void dev_hessian_cholesky(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    int nc = SF(n_con, 0);
+    for (int i = 0; i < D; i++)
+        for (int j = 0; j <= i; j++) SF(nt_H, i * D + j) = 0.f;
+    for (int i = 0; i < D; i++)
+        for (int c = 0; c < nc; c++) {
+            float ji = JAC(c, i);
+            if (fabsf(ji) > m.eps) {
+                float w = SF(efc_D, c) * (float)SF(active, c);
+                for (int j = 0; j <= i; j++) SF(nt_H, i * D + j) = SF(nt_H, i * D + j) + JAC(c, j) * ji * SF(efc_D, c) * (float)SF(active, c);
+                (void)w;
+            }
+        }
+    for (int e = 0; e < m.E; e++)
+        for (int i = m.entity_dof_start[e]; i < m.entity_dof_end[e]; i++)
+            for (int j = m.entity_dof_start[e]; j <= i; j++) SF(nt_H, i * D + j) = SF(nt_H, i * D + j) + SF(mass_mat, i * D + j);
+    for (int i = 0; i < D; i++) {
+        float t = SF(nt_H, i * D + i);
+        for (int j = 0; j < i; j++) t = t - SF(nt_H, i * D + j) * SF(nt_H, i * D + j);
+        float dii = sqrtf(fmaxf(t, m.eps));
+        SF(nt_H, i * D + i) = dii;
+        float inv = 1.f / dii;
+        for (int j = i + 1; j < D; j++) {
+            float dt = 0.f;
+            for (int k = 0; k < i; k++) dt = dt + SF(nt_H, j * D + k) * SF(nt_H, i * D + k);
+            SF(nt_H, j * D + i) = (SF(nt_H, j * D + i) - dt) * inv;
+        }
+    }
+}
+
+// solver.py:1782-1822; returns true when the update degenerates
+__device__ inline bool dev_cholesky_incremental(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    int nc = SF(n_con, 0);
+    for (int c = 0; c < nc; c++) {
+        if (!(SF(active, c) ^ SF(prev_active, c))) continue;
+        float sign = SF(active, c) ? 1.f : -1.f;
+        float sq = sqrtf(SF(efc_D, c));
+        for (int d = 0; d < D; d++) SF(nt_vec, d) = JAC(c, d) * sq;
+        for (int k = 0; k < D; k++) {
+            float vk = SF(nt_vec, k);
+            if (!(fabsf(vk) > m.eps)) continue;
+            float Lkk = SF(nt_H, k * D + k);
+            float t = Lkk * Lkk + sign * vk * vk;
+            if (t < m.eps) return true;
+            float r = sqrtf(t);
+            float cc = r / Lkk, cinv = 1.f / cc, ss = vk / Lkk;
+            SF(nt_H, k * D + k) = r;
+            for (int i = k + 1; i < D; i++) SF(nt_H, i * D + k) = (SF(nt_H, i * D + k) + ss * SF(nt_vec, i) * sign) * cinv;
+            for (int i = k + 1; i < D; i++) SF(nt_vec, i) = SF(nt_vec, i) * cc - ss * SF(nt_H, i * D + k);
+        }
+    }
+    return false;
+}
+
+// solver.py:2722-2751 + :1893-1910
+__device__ inline void dev_gradient(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    for (int d = 0; d < D; d++) SF(grad, d) = SF(Ma, d) - SF(force, d) - SF(qfrc_constraint, d);
+    for (int i = 0; i < D; i++) {
+        float cur = SF(grad, i);
+        for (int j = 0; j < i; j++) cur = cur - SF(nt_H, i * D + j) * SF(Mgrad, j);
+        SF(Mgrad, i) = cur / SF(nt_H, i * D + i);
+    }
+    for (int ii = 0; ii < D; ii++) {
+        int i = D - 1 - ii;
+        float cur = SF(Mgrad, i);
+        for (int j = i + 1; j < D; j++) cur = cur - SF(nt_H, j * D + i) * SF(Mgrad, j);
+        SF(Mgrad, i) = cur / SF(nt_H, i * D + i);
+    }
+}
+
+struct LsP {
+    float alpha, cost, grad, hess;
+};
+
+// solver.py:2034-2152
+__device__ inline LsP dev_ls_init(const DModel& m, const DState& st, int b, int& ls_it) {
+    const int D = m.D;
+    int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
+    for (int e = 0; e < m.E; e++)
+        for (int i = m.entity_dof_start[e]; i < m.entity_dof_end[e]; i++) {
+            float mv = 0.f;
+            for (int j = m.entity_dof_start[e]; j < m.entity_dof_end[e]; j++) mv = mv + SF(mass_mat, i * D + j) * SF(search, j);
+            SF(mv, i) = mv;
+        }
+    for (int c = 0; c < nc; c++) {
+        float jv = 0.f;
+        for (int d = 0; d < D; d++) jv = jv + JAC(c, d) * SF(search, d);
+        SF(jv, c) = jv;
+    }
+    float g1 = 0.f, g2 = 0.f;
+    for (int d = 0; d < D; d++) {
+        float s = SF(search, d);
+        g1 = g1 + (s * SF(Ma, d) - s * SF(force, d));
+        g2 = g2 + 0.5f * s * SF(mv, d);
+    }
+    float gauss = SF(gauss, 0);
+    SF(quad_gauss, 0) = gauss;
+    SF(quad_gauss, 1) = g1;
+    SF(quad_gauss, 2) = g2;
+    float t0 = gauss, t1 = g1, t2 = g2, e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    for (int c = 0; c < nc; c++) {
+        float ja = SF(Jaref, c), jv = SF(jv, c), Dc = SF(efc_D, c);
+        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+        if (c < ne) {
+            e0 = e0 + f0; e1 = e1 + f1; e2 = e2 + f2;
+            t0 = t0 + f0; t1 = t1 + f1; t2 = t2 + f2;
+        } else if (c < nef) {
+            float f = SF(efc_frictionloss, c), r = SF(diag, c), rf = r * f;
+            bool ln = ja <= -rf, lp = ja >= rf;
+            if (ln || lp) {
+                f0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+                f1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+                f2 = 0.f;
+            }
+            t0 = t0 + f0; t1 = t1 + f1; t2 = t2 + f2;
+        } else {
+            float a = (float)(ja < 0.f);
+            t0 = t0 + f0 * a; t1 = t1 + f1 * a; t2 = t2 + f2 * a;
+        }
+    }
+    SF(eq_sum, 0) = e0;
+    SF(eq_sum, 1) = e1;
+    SF(eq_sum, 2) = e2;
+    float h = 2.f * t2;
+    if (h <= 0.f) h = m.eps;
+    ls_it = 1;
+    return {0.f, t0, t1, h};
+}
+
+// solver.py:2155-2355 (n = 1 or 3 candidate steps in one pass over the rows)
+__device__ inline void dev_ls_eval(const DModel& m, const DState& st, int b, const float* al, int n, LsP* out,
+                                   int& ls_it) {
+    int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
+    float b0 = SF(quad_gauss, 0) + SF(eq_sum, 0), b1 = SF(quad_gauss, 1) + SF(eq_sum, 1),
+          b2 = SF(quad_gauss, 2) + SF(eq_sum, 2);
+    float t[3][3];
+    for (int k = 0; k < 3; k++) {
+        t[k][0] = b0;
+        t[k][1] = b1;
+        t[k][2] = b2;
+    }
+    for (int c = ne; c < nef; c++) {
+        float ja = SF(Jaref, c), jv = SF(jv, c), Dc = SF(efc_D, c), f = SF(efc_frictionloss, c), r = SF(diag, c);
+        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv), rf = r * f;
+        for (int k = 0; k < n; k++) {
+            float x = ja + al[k] * jv;
+            bool ln = x <= -rf, lp = x >= rf;
+            float a0 = f0, a1 = f1, a2 = f2;
+            if (ln || lp) {
+                a0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+                a1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+                a2 = 0.f;
+            }
+            t[k][0] = t[k][0] + a0; t[k][1] = t[k][1] + a1; t[k][2] = t[k][2] + a2;
+        }
+    }
+    for (int c = nef; c < nc; c++) {
+        float ja = SF(Jaref, c), jv = SF(jv, c), Dc = SF(efc_D, c);
+        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+        for (int k = 0; k < n; k++) {
+            float a = (float)(ja + al[k] * jv < 0.f);
+            t[k][0] = t[k][0] + f0 * a; t[k][1] = t[k][1] + f1 * a; t[k][2] = t[k][2] + f2 * a;
+        }
+    }
+    for (int k = 0; k < n; k++) {
+        float a = al[k];
+        float h = 2.f * t[k][2];
+        if (h <= 0.f) h = m.eps;
+        out[k] = {a, a * a * t[k][2] + a * t[k][1] + t[k][0], 2.f * a * t[k][2] + t[k][1], h};
+    }
+    ls_it += n;
+}
+
+__device__ inline int bracket(LsP& p, const LsP* c, float& next) {
+    int flag = 0;
+    for (int i = 0; i < 3; i++) {
+        if (p.grad < 0.f && c[i].grad < 0.f && p.grad < c[i].grad) {
+            p = c[i];
+            flag = 1;
+        } else if (p.grad > 0.f && c[i].grad > 0.f && p.grad > c[i].grad) {
+            p = c[i];
+            flag = 2;
+        }
+    }
+    next = p.alpha;
+    if (flag > 0) next = p.alpha - p.grad / p.hess;
+    return flag;
+}
+
+// solver.py:2427-2597 func_linesearch_batch
+__device__ inline float dev_linesearch(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    float sn = 0.f;
+    for (int d = 0; d < D; d++) sn = sn + SF(search, d) * SF(search, d);
+    sn = sqrtf(sn);
+    float gtol = m.tolerance * m.ls_tolerance * sn * (m.meaninertia * (float)max(1, D));
+    SF(gtol, 0) = gtol;
+    int ls_it = 0, res_code = 0;
+    float res = 0.f;
+    if (sn < m.eps) {
+        SF(ls_result, 0) = 1;
+        SF(ls_it, 0) = 0;
+        return 0.f;
+    }
+    LsP p0 = dev_ls_init(m, st, b, ls_it);
+    LsP p1;
+    float a = p0.alpha - p0.grad / p0.hess;
+    dev_ls_eval(m, st, b, &a, 1, &p1, ls_it);
+    if (p0.cost < p1.cost) p1 = p0;
+    bool done = false;
+    if (fabsf(p1.grad) < gtol) {
+        res_code = fabsf(p1.alpha) < m.eps ? 2 : 0;
+        res = p1.alpha;
+        done = true;
+    }
+    LsP p2 = p1;
+    int p2update = 0;
+    if (!done) {
+        float dir = (float)((p1.grad < 0.f) ? 1 : 0) * 2.f - 1.f;
+        while (p1.grad * dir <= -gtol && ls_it < m.ls_iterations) {
+            p2 = p1;
+            p2update = 1;
+            float an = p1.alpha - p1.grad / p1.hess;
+            dev_ls_eval(m, st, b, &an, 1, &p1, ls_it);
+            if (fabsf(p1.grad) < gtol) {
+                res = p1.alpha;
+                done = true;
+                break;
+            }
+        }
+    }
+    if (!done && ls_it >= m.ls_iterations) {
+        res_code = 3;
+        res = p1.alpha;
+        done = true;
+    }
+    if (!done && !p2update) {
+        res_code = 6;
+        res = p1.alpha;
+        done = true;
+    }
+    if (!done) {
+        float al[3] = {p1.alpha - p1.grad / p1.hess, p1.alpha, (p1.alpha + p2.alpha) * 0.5f};
+        while (ls_it < m.ls_iterations) {
+            LsP c[3];
+            dev_ls_eval(m, st, b, al, 3, c, ls_it);
+            float n1 = al[0], n2 = al[1];
+            bool found = false;
+            float ba = 0.f, bc = 0.f;
+            for (int i = 0; i < 3; i++)
+                if (fabsf(c[i].grad) < gtol && (!found || c[i].cost < bc)) {
+                    ba = c[i].alpha;
+                    bc = c[i].cost;
+                    found = true;
+                }
+            if (found) {
+                res = ba;
+                done = true;
+                break;
+            }
+            int b1 = bracket(p1, c, n1), b2 = bracket(p2, c, n2);
+            if (b1 == 0 && b2 == 0) {
+                res_code = c[2].cost < p0.cost ? 0 : 7;
+                res = al[2];
+                done = true;
+                break;
+            }
+            al[0] = n1;
+            al[1] = n2;
+            al[2] = (p1.alpha + p2.alpha) * 0.5f;
+        }
+        if (!done) {
+            if (p1.cost <= p2.cost && p1.cost < p0.cost) {
+                res_code = 4;
+                res = p1.alpha;
+            } else if (p2.cost <= p1.cost && p2.cost < p0.cost) {
+                res_code = 4;
+                res = p2.alpha;
+            } else {
+                res_code = 5;
+                res = 0.f;
+            }
+        }
+    }
+    SF(ls_result, 0) = res_code;
+    SF(ls_it, 0) = ls_it;
+    return res;
+}
+
+// solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
+__device__ inline void dev_solve(const DModel& m, const DState& st, int b) {
+    const int D = m.D;
+    int nc = SF(n_con, 0);
+    bool ws = nc > 0 && SF(is_warmstart, 0);
+    for (int d = 0; d < D; d++) SF(qacc, d) = ws ? SF(qacc_ws, d) : SF(acc_smooth, d);
+    for (int d1 = 0; d1 < D; d1++) {
+        int e = m.dof_entity[d1];
+        float ma = 0.f;
+        for (int d2 = m.entity_dof_start[e]; d2 < m.entity_dof_end[e]; d2++) ma = ma + SF(mass_mat, d1 * D + d2) * SF(qacc, d2);
+        SF(Ma, d1) = ma;
+    }
+    for (int c = 0; c < nc; c++) {
+        float ja = -SF(aref, c);
+        for (int d = 0; d < D; d++) ja = ja + JAC(c, d) * SF(qacc, d);
+        SF(Jaref, c) = ja;
+    }
+    float cost = dev_update_constraint(m, st, b, SF(cost, 0));
+    SF(cost, 0) = cost;
+    int improved = nc > 0;
+    dev_hessian_cholesky(m, st, b);
+    dev_gradient(m, st, b);
+    for (int d = 0; d < D; d++) SF(search, d) = -SF(Mgrad, d);
+    int n_iter = 0;
+    if (nc > 0) {
+        const float tol = (m.meaninertia * (float)max(1, D)) * m.tolerance;
+        for (int it = 0; it < m.iterations; it++) {
+            n_iter++;
+            float alpha = dev_linesearch(m, st, b);
+            if (fabsf(alpha) < m.eps) {
+                improved = 0;
+            } else {
+                for (int d = 0; d < D; d++) {
+                    SF(qacc, d) = SF(qacc, d) + SF(search, d) * alpha;
+                    SF(Ma, d) = SF(Ma, d) + SF(mv, d) * alpha;
+                }
+                for (int c = 0; c < nc; c++) SF(Jaref, c) = SF(Jaref, c) + SF(jv, c) * alpha;
+                float prev = SF(cost, 0);
+                cost = dev_update_constraint(m, st, b, prev);
+                SF(cost, 0) = cost;
+                if (dev_cholesky_incremental(m, st, b)) dev_hessian_cholesky(m, st, b);
+                dev_gradient(m, st, b);
+                float gn = 0.f;
+                for (int d = 0; d < D; d++) gn = gn + SF(grad, d) * SF(grad, d);
+                gn = sqrtf(gn);
+                improved = gn > tol && (prev - cost) > tol;
+                if (improved)
+                    for (int d = 0; d < D; d++) SF(search, d) = -SF(Mgrad, d);
+            }
+            if (!improved) break;
+        }
+    } else {
+        improved = 0;
+    }
+    SF(improved, 0) = improved;
+    SF(n_iter, 0) = n_iter;
+    for (int d = 0; d < D; d++) {
+        float qa = SF(qacc, d);
+        SF(acc, d) = qa;
+        SF(qf_constraint, d) = SF(qfrc_constraint, d);
+        SF(force, d) = SF(qf_smooth, d) + SF(qfrc_constraint, d);
+        SF(qacc_ws, d) = qa;
+        if (isnan(qa)) SF(errbits, 0) = SF(errbits, 0) | RS_ERR_INVALID_FORCE_NAN;
+    }
+    SF(is_warmstart, 0) = 1;
+    int cs = SF(n_con_eq, 0) + SF(n_con_fl, 0);
+    int ncont = SF(n_contacts, 0);
+    for (int i = 0; i < ncont; i++) {
+        f3 nrm = LD3(contact_normal, i);
+        float fr = SF(contact_friction, i);
+        f3 d1, d2, f = zero3();
+        orthogonals(nrm, d1, d2);
+        for (int k = 0; k < 4; k++) {
+            f3 dd = (float)(2 * (k % 2) - 1) * (k < 2 ? d1 : d2);
+            f = f + (dd * fr - nrm) * SF(efc_force, i * 4 + k + cs);
+        }
+        ST3(contact_force, i, f);
+    }
+}
+
+#undef JAC
+
+}  // namespace rs

@@ -1,0 +1,416 @@
+// librigidstep: C ABI (include/rigidstep.h) over the gfx950 rigid-step kernels.
+//
+// Plain path: one kernel per stage of RigidSolver.substep (rigid_solver.py:912-980), one lane per environment, every
+// intermediate kept in the [row][env] planes of the state slab (the comparison point and drop-in).
+// Fused path: the whole substep in a single kernel per launch (rs_fused.hip).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rs_collision.h"
+#include "rs_constraint.h"
+#include "rs_dyn.h"
+#include "rs_fused.h"
+
+using namespace rs;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(RS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------ directories
+namespace {
+struct Info {
+    const char* name;
+    int is_int, count, width;
+};
+const Info k_model_arrays[] = {
+#define RS_X(name, is_int, kind, width) {#name, is_int, kind, width},
+    RS_MODEL_ARRAYS(RS_X)
+#undef RS_X
+};
+const Info k_state_fields[] = {
+#define RS_X(name, is_int, kind, width) {#name, is_int, kind, width},
+    RS_STATE_FIELDS(RS_X)
+#undef RS_X
+};
+constexpr int k_n_model_arrays = sizeof(k_model_arrays) / sizeof(Info);
+constexpr int k_n_state_fields = sizeof(k_state_fields) / sizeof(Info);
+}  // namespace
+
+struct rs_model {
+    rs_model_header h;
+    void* blob = nullptr;
+    size_t blob_bytes = 0;
+    DModel dm;
+    int device = 0;
+};
+
+struct rs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int64_t count_of(const rs_model_header& h, int kind) {
+    switch (kind) {
+        case RS_N_ONE: return 1;
+        case RS_N_LINKS: return h.n_links;
+        case RS_N_JOINTS: return h.n_joints;
+        case RS_N_DOFS: return h.n_dofs;
+        case RS_N_QS: return h.n_qs;
+        case RS_N_GEOMS: return h.n_geoms;
+        case RS_N_ENTITIES: return h.n_entities;
+        case RS_N_EQUALITIES: return h.n_equalities;
+        case RS_N_DOFS2: return (int64_t)h.n_dofs * h.n_dofs;
+        case RS_N_GEOMS2: return (int64_t)h.n_geoms * h.n_geoms;
+        case RS_N_PAIRS: return h.n_pairs;
+        case RS_N_SUPPORT: return h.n_support;
+        case RS_N_CONTACTS: return h.max_contact_pairs;
+        case RS_N_CONSTRAINTS: return h.len_constraints;
+        case RS_N_CONSTRAINTS_DOFS: return (int64_t)h.len_constraints * h.n_dofs;
+        case RS_N_BROAD: return h.max_collision_pairs_broad;
+        case RS_N_SORT: return 2 * (int64_t)h.n_geoms;
+        default: return 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ state views
+static void layout(const rs_model_header& h, int n_envs, int64_t* offs, int64_t* total) {
+    int64_t off = 0;
+    for (int i = 0; i < k_n_state_fields; i++) {
+        int64_t rows = count_of(h, k_state_fields[i].count) * k_state_fields[i].width;
+        if (rows < 1) rows = 1;  // keep every plane addressable even for empty models
+        offs[i] = off;
+        off += (rows * n_envs * 4 + 255) / 256 * 256;
+    }
+    *total = off;
+}
+
+static DState make_state(const rs_model* m, void* base, int n_envs) {
+    std::vector<int64_t> offs(k_n_state_fields);
+    int64_t total;
+    layout(m->h, n_envs, offs.data(), &total);
+    DState s;
+    s.B = n_envs;
+    int i = 0;
+    char* p = (char*)base;
+#define RS_X(name, is_int, kind, width) s.name = (decltype(s.name))(p + offs[i++]);
+    RS_STATE_FIELDS(RS_X)
+#undef RS_X
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------- kernels
+static constexpr int kBlock = 64;
+
+__global__ void __launch_bounds__(kBlock) k_init(DModel m, DState st) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= st.B) return;
+    for (int q = 0; q < m.Q; q++) SF(qpos, q) = m.qpos0[q];
+    for (int d = 0; d < m.D; d++) SF(ctrl_mode, d) = CTRL_FORCE;
+    SF(first_time, 0) = 1;
+    for (int l = 0; l < m.L; l++) {
+        ST3(links_pos, l, mvec3(m.link_pos, l));
+        ST4(links_quat, l, mquat(m.link_quat, l));
+    }
+    dev_kinematics(m, st, b, true);
+}
+
+__global__ void __launch_bounds__(kBlock) k_fk_masked(DModel m, DState st, const int32_t* mask) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= st.B) return;
+    if (mask && !mask[b]) return;
+    dev_kinematics(m, st, b, false);
+}
+
+__global__ void __launch_bounds__(kBlock) k_reset_caches(DModel m, DState st, const int32_t* mask) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= st.B) return;
+    if (mask && !mask[b]) return;
+    for (int d = 0; d < m.D; d++) SF(qacc_ws, d) = 0.f;
+    SF(is_warmstart, 0) = 0;
+    for (int p = 0; p < 3 * m.P; p++) SF(cache_normal, p) = 0.f;
+    SF(first_time, 0) = 1;
+    SF(errbits, 0) = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_stage(DModel m, DState st, int stage) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= st.B) return;
+    switch (stage) {
+        case RS_STAGE_FK: dev_kinematics(m, st, b, false); break;
+        case RS_STAGE_CRBA: dev_mass_matrix(m, st, b); break;
+        case RS_STAGE_FACTOR: dev_factor_mass(m, st, b); break;
+        case RS_STAGE_RNE:
+            dev_torque_passive(m, st, b);
+            dev_update_acc(m, st, b, false);
+            dev_rne_and_qacc(m, st, b);
+            break;
+        case RS_STAGE_BROADPHASE: dev_broadphase(m, st, b); break;
+        case RS_STAGE_NARROWPHASE: dev_narrowphase(m, st, b); break;
+        case RS_STAGE_CONSTRAINTS:
+            dev_equality_constraints(m, st, b);
+            dev_inequality_constraints(m, st, b);
+            break;
+        case RS_STAGE_SOLVE: dev_solve(m, st, b); break;
+        case RS_STAGE_INTEGRATE:
+            dev_update_acc(m, st, b, true);
+            dev_integrate(m, st, b);
+            dev_kinematics(m, st, b, false);
+            break;
+        default: break;
+    }
+}
+
+// one lane per (geom, env)
+__global__ void __launch_bounds__(256) k_aabb(DModel m, DState st) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.G * st.B) return;
+    int g = (int)(t / st.B), b = (int)(t % st.B);
+    dev_aabb(m, st, b, g);
+}
+
+__global__ void k_errno_or(DState st, int32_t* out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = b < st.B ? st.errbits[b] : 0;
+    if (v) atomicOr(out, v);
+}
+
+// ------------------------------------------------------------------------------------------------ launch helpers
+static inline dim3 env_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
+
+static int launch_stage(const rs_model* m, const DState& st, int stage, hipStream_t s) {
+    const DModel& dm = m->dm;
+    if (stage == RS_STAGE_AABB) {
+        int64_t n = (int64_t)dm.G * st.B;
+        if (n > 0) hipLaunchKernelGGL(k_aabb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dm, st);
+    } else if ((stage == RS_STAGE_BROADPHASE || stage == RS_STAGE_NARROWPHASE) && dm.P == 0) {
+        // no candidate pairs: detection stops after the AABBs (collider.py:690-691)
+    } else if ((stage == RS_STAGE_CONSTRAINTS || stage == RS_STAGE_SOLVE) && dm.disable_constraint) {
+    } else {
+        hipLaunchKernelGGL(k_stage, env_grid(st.B), dim3(kBlock), 0, s, dm, st, stage);
+    }
+    HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+static int plain_substep(const rs_model* m, const DState& st, hipStream_t s) {
+    const int order[] = {RS_STAGE_CRBA, RS_STAGE_FACTOR, RS_STAGE_RNE};
+    for (int stg : order)
+        if (int r = launch_stage(m, st, stg, s)) return r;
+    if (m->dm.enable_collision) {
+        for (int stg : {RS_STAGE_AABB, RS_STAGE_BROADPHASE, RS_STAGE_NARROWPHASE})
+            if (int r = launch_stage(m, st, stg, s)) return r;
+    }
+    if (!m->dm.disable_constraint) {
+        for (int stg : {RS_STAGE_CONSTRAINTS, RS_STAGE_SOLVE})
+            if (int r = launch_stage(m, st, stg, s)) return r;
+    }
+    return launch_stage(m, st, RS_STAGE_INTEGRATE, s);
+}
+
+// ------------------------------------------------------------------------------------------------------- ABI
+extern "C" {
+
+const char* rs_last_error(void) { return g_err.c_str(); }
+int rs_abi_version(void) { return RS_ABI_VERSION; }
+
+int rs_model_array_count(void) { return k_n_model_arrays; }
+int rs_model_array_info(int i, rs_array_info* out) {
+    if (i < 0 || i >= k_n_model_arrays || !out) return fail(RS_E_INVALID, "rs_model_array_info: bad index");
+    out->name = k_model_arrays[i].name;
+    out->is_int = k_model_arrays[i].is_int;
+    out->count = k_model_arrays[i].count;
+    out->width = k_model_arrays[i].width;
+    return RS_OK;
+}
+int rs_state_field_count(void) { return k_n_state_fields; }
+int rs_state_field_info(int i, rs_array_info* out) {
+    if (i < 0 || i >= k_n_state_fields || !out) return fail(RS_E_INVALID, "rs_state_field_info: bad index");
+    out->name = k_state_fields[i].name;
+    out->is_int = k_state_fields[i].is_int;
+    out->count = k_state_fields[i].count;
+    out->width = k_state_fields[i].width;
+    return RS_OK;
+}
+
+int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_bytes, rs_model** out) {
+    if (!h || !blob || !out) return fail(RS_E_INVALID, "rs_model_create: null argument");
+    if (h->abi_version != RS_ABI_VERSION) return fail(RS_E_VERSION, "rs_model_create: ABI version mismatch");
+    if (h->float_bytes != 4) return fail(RS_E_INVALID, "rs_model_create: device blob must hold float32");
+    if (h->n_arrays != k_n_model_arrays) return fail(RS_E_VERSION, "rs_model_create: array directory mismatch");
+    if (h->n_dofs < 0 || h->n_links < 1 || h->n_geoms < 0) return fail(RS_E_INVALID, "rs_model_create: bad counts");
+    for (int i = 0; i < k_n_model_arrays; i++) {
+        int64_t n = count_of(*h, k_model_arrays[i].count) * k_model_arrays[i].width * 4;
+        if (h->array_offset[i] < 0 || h->array_offset[i] + n > (int64_t)blob_bytes)
+            return fail(RS_E_INVALID, std::string("rs_model_create: array out of blob: ") + k_model_arrays[i].name);
+    }
+    rs_model* m = new rs_model();
+    m->h = *h;
+    HIP_OK(hipGetDevice(&m->device));
+    m->blob_bytes = blob_bytes;
+    hipError_t e = hipMalloc(&m->blob, blob_bytes);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(RS_E_DEVICE, std::string("hipMalloc model: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpy(m->blob, blob, blob_bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(m->blob);
+        delete m;
+        return fail(RS_E_DEVICE, std::string("hipMemcpy model: ") + hipGetErrorString(e));
+    }
+    DModel& d = m->dm;
+    d.L = h->n_links; d.J = h->n_joints; d.D = h->n_dofs; d.Q = h->n_qs; d.G = h->n_geoms; d.E = h->n_entities;
+    d.NE = h->n_equalities; d.P = h->n_pairs; d.NSUP = h->n_support; d.KMAX = h->max_contact_pairs;
+    d.CMAX = h->len_constraints; d.BROAD = h->max_collision_pairs_broad; d.NCPP = h->n_contacts_per_pair;
+    d.dt = h->dt; d.gx = h->gravity[0]; d.gy = h->gravity[1]; d.gz = h->gravity[2];
+    d.tolerance = h->tolerance; d.ls_tolerance = h->ls_tolerance; d.meaninertia = h->meaninertia;
+    d.mc_perturbation = h->mc_perturbation; d.mc_tolerance = h->mc_tolerance;
+    d.eps = 1.1920928955078125e-07f;
+    d.iterations = h->iterations; d.ls_iterations = h->ls_iterations;
+    d.enable_collision = h->enable_collision; d.enable_joint_limit = h->enable_joint_limit;
+    d.disable_constraint = h->disable_constraint; d.enable_multi_contact = h->enable_multi_contact;
+    d.box_box_detection = h->box_box_detection; d.integrator = h->integrator; d.solver_type = h->solver_type;
+    int i = 0;
+    const char* base = (const char*)m->blob;
+#define RS_X(name, is_int, kind, width) d.name = (decltype(d.name))(base + h->array_offset[i++]);
+    RS_MODEL_ARRAYS(RS_X)
+#undef RS_X
+    if (h->solver_type != 1) {
+        rs_model_destroy(m);
+        return fail(RS_E_UNSUPPORTED, "rs_model_create: only the Newton constraint solver is built");
+    }
+    if (h->integrator != 2) {
+        rs_model_destroy(m);
+        return fail(RS_E_UNSUPPORTED, "rs_model_create: only the approximate_implicitfast integrator is built");
+    }
+    *out = m;
+    return RS_OK;
+}
+
+void rs_model_destroy(rs_model* m) {
+    if (!m) return;
+    if (m->blob) (void)hipFree(m->blob);
+    delete m;
+}
+
+int rs_state_layout(const rs_model* m, int n_envs, int64_t* offsets, int64_t* total_bytes) {
+    if (!m || n_envs < 1 || !total_bytes) return fail(RS_E_INVALID, "rs_state_layout: bad argument");
+    std::vector<int64_t> offs(k_n_state_fields);
+    layout(m->h, n_envs, offs.data(), total_bytes);
+    if (offsets) std::memcpy(offsets, offs.data(), sizeof(int64_t) * k_n_state_fields);
+    return RS_OK;
+}
+
+int rs_state_init(const rs_model* m, void* state, int n_envs, void* stream) {
+    if (!m || !state || n_envs < 1) return fail(RS_E_INVALID, "rs_state_init: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int64_t total;
+    std::vector<int64_t> offs(k_n_state_fields);
+    layout(m->h, n_envs, offs.data(), &total);
+    HIP_OK(hipMemsetAsync(state, 0, total, s));
+    DState st = make_state(m, state, n_envs);
+    hipLaunchKernelGGL(k_init, env_grid(n_envs), dim3(kBlock), 0, s, m->dm, st);
+    HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+int rs_step(const rs_model* m, void* state, int n_envs, int n_substeps, int mode, void* stream) {
+    if (!m || !state || n_envs < 1 || n_substeps < 0) return fail(RS_E_INVALID, "rs_step: bad argument");
+    if (mode != RS_MODE_PLAIN && mode != RS_MODE_FUSED) return fail(RS_E_INVALID, "rs_step: bad mode");
+    hipStream_t s = (hipStream_t)stream;
+    DState st = make_state(m, state, n_envs);
+    for (int k = 0; k < n_substeps; k++) {
+        int r = mode == RS_MODE_PLAIN ? plain_substep(m, st, s) : fused_substep(m->dm, st, s);
+        if (r) return r == RS_E_DEVICE ? fail(r, std::string("launch failed: ") + hipGetErrorString(hipGetLastError())) : r;
+    }
+    return RS_OK;
+}
+
+int rs_stage(const rs_model* m, void* state, int n_envs, int stage, void* stream) {
+    if (!m || !state || n_envs < 1 || stage < 0 || stage >= RS_STAGE_COUNT) return fail(RS_E_INVALID, "rs_stage: bad argument");
+    DState st = make_state(m, state, n_envs);
+    return launch_stage(m, st, stage, (hipStream_t)stream);
+}
+
+int rs_forward_kinematics(const rs_model* m, void* state, int n_envs, const int32_t* mask, void* stream) {
+    if (!m || !state || n_envs < 1) return fail(RS_E_INVALID, "rs_forward_kinematics: bad argument");
+    DState st = make_state(m, state, n_envs);
+    hipLaunchKernelGGL(k_fk_masked, env_grid(n_envs), dim3(kBlock), 0, (hipStream_t)stream, m->dm, st, mask);
+    HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+int rs_reset_caches(const rs_model* m, void* state, int n_envs, const int32_t* mask, void* stream) {
+    if (!m || !state || n_envs < 1) return fail(RS_E_INVALID, "rs_reset_caches: bad argument");
+    DState st = make_state(m, state, n_envs);
+    hipLaunchKernelGGL(k_reset_caches, env_grid(n_envs), dim3(kBlock), 0, (hipStream_t)stream, m->dm, st, mask);
+    HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+int rs_errno_reduce(const rs_model* m, const void* state, int n_envs, int32_t* out, void* stream) {
+    if (!m || !state || n_envs < 1 || !out) return fail(RS_E_INVALID, "rs_errno_reduce: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    DState st = make_state(m, (void*)state, n_envs);
+    int32_t* dv = nullptr;
+    HIP_OK(hipMallocAsync((void**)&dv, sizeof(int32_t), s));
+    HIP_OK(hipMemsetAsync(dv, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_errno_or, dim3((n_envs + 255) / 256), dim3(256), 0, s, st, dv);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, dv, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipFreeAsync(dv, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return RS_OK;
+}
+
+int rs_graph_create(const rs_model* m, void* state, int n_envs, int n_substeps, int mode, void* stream, rs_graph** out) {
+    if (!m || !state || !out || n_envs < 1 || n_substeps < 1) return fail(RS_E_INVALID, "rs_graph_create: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) return fail(RS_E_INVALID, "rs_graph_create: capture needs a non-default stream");
+    rs_graph* g = new rs_graph();
+    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = rs_step(m, state, n_envs, n_substeps, mode, stream);
+    hipError_t e = hipStreamEndCapture(s, &g->graph);
+    if (r != RS_OK || e != hipSuccess) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return r != RS_OK ? r : fail(RS_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g->graph);
+        delete g;
+        return fail(RS_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return RS_OK;
+}
+
+int rs_graph_launch(rs_graph* g, void* stream) {
+    if (!g) return fail(RS_E_INVALID, "rs_graph_launch: null graph");
+    HIP_OK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return RS_OK;
+}
+
+void rs_graph_destroy(rs_graph* g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+}  // extern "C"

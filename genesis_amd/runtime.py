@@ -1,0 +1,225 @@
+"""Host runtime over the C ABI of librigidstep.so (include/rigidstep.h).
+
+PyTorch (ROCm) owns all device memory: the per-environment state slab is one uint8 tensor, and every state field is
+exposed as a ``[rows, n_envs]`` float32/int32 view into it, i.e. the reference's ``[n, B]`` field layout
+(genesis/utils/array_class.py). The library borrows raw pointers and enqueues work on torch's current stream.
+There is no CPU fallback: if the library cannot be loaded, everything here raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from .model.blob import ModelHeader, pack_model
+from .model.compiler import CompiledModel
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librigidstep.so")
+
+RS_MODE_PLAIN, RS_MODE_FUSED = 0, 1
+STAGES = ["fk", "crba", "factor", "rne", "aabb", "broadphase", "narrowphase", "constraints", "solve", "integrate"]
+STAGE_ID = {n: i for i, n in enumerate(STAGES)}
+
+ERR_OVERFLOW_CANDIDATE_CONTACTS = 1 << 0
+ERR_OVERFLOW_COLLISION_PAIRS = 1 << 1
+ERR_INVALID_FORCE_NAN = 1 << 2
+ERR_INVALID_ACC_NAN = 1 << 3
+
+COUNT_KINDS = [
+    "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
+    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT",
+]
+
+
+class RigidStepError(RuntimeError):
+    pass
+
+
+class _ArrayInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("is_int", ctypes.c_int32), ("count", ctypes.c_int32),
+                ("width", ctypes.c_int32)]
+
+
+_lib = None
+
+# every exported entry point with (restype, argtypes)
+_SIGNATURES = {
+    "rs_abi_version": (ctypes.c_int, []),
+    "rs_last_error": (ctypes.c_char_p, []),
+    "rs_model_array_count": (ctypes.c_int, []),
+    "rs_model_array_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_ArrayInfo)]),
+    "rs_state_field_count": (ctypes.c_int, []),
+    "rs_state_field_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_ArrayInfo)]),
+    "rs_model_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "rs_model_destroy": (None, [ctypes.c_void_p]),
+    "rs_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_int64)]),
+    "rs_state_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "rs_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_void_p]),
+    "rs_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "rs_forward_kinematics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "rs_reset_caches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
+    "rs_errno_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
+    "rs_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "rs_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "rs_graph_destroy": (None, [ctypes.c_void_p]),
+}
+
+
+def load_library(path: str | None = None):
+    """Load librigidstep.so (raises if it is missing: there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RigidStepError(f"{path} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.rs_abi_version() != 1:
+        raise RigidStepError("librigidstep ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def _check(code: int):
+    if code != 0:
+        raise RigidStepError(f"librigidstep error {code}: {_lib.rs_last_error().decode()}")
+
+
+def state_field_directory():
+    lib = load_library()
+    out = []
+    info = _ArrayInfo()
+    for i in range(lib.rs_state_field_count()):
+        _check(lib.rs_state_field_info(i, ctypes.byref(info)))
+        out.append((info.name.decode(), bool(info.is_int), COUNT_KINDS[info.count], info.width))
+    return out
+
+
+def model_array_directory():
+    lib = load_library()
+    out = []
+    info = _ArrayInfo()
+    for i in range(lib.rs_model_array_count()):
+        _check(lib.rs_model_array_info(i, ctypes.byref(info)))
+        out.append((info.name.decode(), bool(info.is_int), COUNT_KINDS[info.count], info.width))
+    return out
+
+
+def _stream_handle(stream=None):
+    s = stream or torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class DeviceModel:
+    """A compiled model uploaded to the current device (``rs_model``)."""
+
+    def __init__(self, model: CompiledModel):
+        lib = load_library()
+        self.model = model
+        self.header, blob = pack_model(model, np.float32)
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        h = ctypes.c_void_p()
+        _check(lib.rs_model_create(ctypes.byref(self.header), buf, len(blob), ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rs_model_destroy(self.handle)
+            self.handle = None
+
+    def count(self, kind: str) -> int:
+        m = self.model
+        return {
+            "ONE": 1, "LINKS": m.n_links, "JOINTS": m.n_joints, "DOFS": m.n_dofs, "QS": m.n_qs, "GEOMS": m.n_geoms,
+            "ENTITIES": m.n_entities, "EQUALITIES": m.n_equalities, "DOFS2": m.n_dofs**2, "GEOMS2": m.n_geoms**2,
+            "PAIRS": m.n_pairs, "SUPPORT": len(m.arrays["support_v"]), "CONTACTS": m.max_contact_pairs,
+            "CONSTRAINTS": m.len_constraints, "CONSTRAINTS_DOFS": m.len_constraints * m.n_dofs,
+            "BROAD": m.max_collision_pairs_broad, "SORT": 2 * m.n_geoms,
+        }[kind]
+
+
+class StateSlab:
+    """Per-environment state of ``n_envs`` environments in one torch-owned device buffer."""
+
+    def __init__(self, dmodel: DeviceModel, n_envs: int, device=None):
+        lib = load_library()
+        self.dmodel = dmodel
+        self.n_envs = n_envs
+        self.directory = state_field_directory()
+        offs = (ctypes.c_int64 * len(self.directory))()
+        total = ctypes.c_int64()
+        _check(lib.rs_state_layout(dmodel.handle, n_envs, offs, ctypes.byref(total)))
+        self.nbytes = total.value
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        self.buffer = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.fields = {}
+        for (name, is_int, kind, width), off in zip(self.directory, offs):
+            rows = max(dmodel.count(kind) * width, 1)
+            raw = self.buffer[off : off + rows * n_envs * 4]
+            self.fields[name] = raw.view(torch.int32 if is_int else torch.float32).view(rows, n_envs)
+        self.init()
+
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(self.buffer.data_ptr())
+
+    def __getitem__(self, name: str) -> torch.Tensor:
+        return self.fields[name]
+
+    def init(self, stream=None):
+        _check(_lib.rs_state_init(self.dmodel.handle, self.ptr, self.n_envs, _stream_handle(stream)))
+
+    def step(self, n_substeps: int = 1, mode: int = RS_MODE_PLAIN, stream=None):
+        _check(_lib.rs_step(self.dmodel.handle, self.ptr, self.n_envs, n_substeps, mode, _stream_handle(stream)))
+
+    def stage(self, name: str, stream=None):
+        _check(_lib.rs_stage(self.dmodel.handle, self.ptr, self.n_envs, STAGE_ID[name], _stream_handle(stream)))
+
+    def forward_kinematics(self, env_mask: torch.Tensor | None = None, stream=None):
+        mp = ctypes.c_void_p(env_mask.data_ptr()) if env_mask is not None else None
+        _check(_lib.rs_forward_kinematics(self.dmodel.handle, self.ptr, self.n_envs, mp, _stream_handle(stream)))
+
+    def reset_caches(self, env_mask: torch.Tensor | None = None, stream=None):
+        mp = ctypes.c_void_p(env_mask.data_ptr()) if env_mask is not None else None
+        _check(_lib.rs_reset_caches(self.dmodel.handle, self.ptr, self.n_envs, mp, _stream_handle(stream)))
+
+    def errno(self, stream=None) -> int:
+        out = ctypes.c_int32()
+        _check(_lib.rs_errno_reduce(self.dmodel.handle, self.ptr, self.n_envs, ctypes.byref(out),
+                                    _stream_handle(stream)))
+        return out.value
+
+
+class StepGraph:
+    """hipGraph of ``n_substeps`` substeps of a slab, replayed with ``launch()``."""
+
+    def __init__(self, slab: StateSlab, n_substeps: int, mode: int, stream: torch.cuda.Stream):
+        self.slab = slab
+        self.stream = stream
+        h = ctypes.c_void_p()
+        _check(_lib.rs_graph_create(slab.dmodel.handle, slab.ptr, slab.n_envs, n_substeps, mode,
+                                    _stream_handle(stream), ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        _check(_lib.rs_graph_launch(self.handle, _stream_handle(self.stream)))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rs_graph_destroy(self.handle)
+            self.handle = None

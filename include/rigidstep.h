@@ -1,0 +1,163 @@
+/*
+ * rigidstep.h - C ABI of the MI355X batched rigid-body step (librigidstep.so).
+ *
+ * The reference exposes this path only as a Python object (RigidSolver, genesis/engine/solvers/rigid/
+ * rigid_solver.py:221) driven by Simulator.step -> RigidSolver.substep (simulator.py:283-286,
+ * rigid_solver.py:912-980). These entry points are what a binding of that object needs: build the static model once
+ * (RigidSolver.build, rigid_solver.py:317-355), describe/initialise the per-environment state slab
+ * (array_class.DataManager, utils/array_class.py:2058), run substeps (substep), run single stages for parity
+ * (kernel_step_1/kernel_step_2 pieces, rigid_solver.py:2719-2891; Collider.detection, collider/collider.py:684;
+ * ConstraintSolver.resolve, constraint/solver.py:210), and reduce the per-env error bits (check_errno,
+ * rigid_solver.py:985-1009).
+ *
+ * Memory: the caller owns all device memory. The model blob is copied to the device by rs_model_create and owned by
+ * the returned handle. The state slab is a caller-allocated device buffer of rs_state_layout().total_bytes bytes whose
+ * fields are [rows][n_envs] planes (environment index fastest), float32 or int32. All work is enqueued on the stream
+ * passed by the caller; no entry point synchronises except rs_errno_reduce and the *_sync helpers.
+ *
+ * Return codes: 0 on success, negative on error (RS_E_*). Error text: rs_last_error().
+ */
+#ifndef RIGIDSTEP_H
+#define RIGIDSTEP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RS_ABI_VERSION 1
+
+enum {
+    RS_OK = 0,
+    RS_E_INVALID = -1,   /* bad argument */
+    RS_E_DEVICE = -2,    /* HIP runtime error */
+    RS_E_VERSION = -3,   /* blob / ABI version mismatch */
+    RS_E_UNSUPPORTED = -4,
+};
+
+/* Per-environment error bits; values equal the reference ErrorCode (utils/array_class.py:91-101). */
+enum {
+    RS_ERR_OVERFLOW_CANDIDATE_CONTACTS = 1 << 0,
+    RS_ERR_OVERFLOW_COLLISION_PAIRS = 1 << 1,
+    RS_ERR_INVALID_FORCE_NAN = 1 << 2,
+    RS_ERR_INVALID_ACC_NAN = 1 << 3,
+    RS_ERR_OVERFLOW_HIBERNATION_ISLANDS = 1 << 4,
+};
+
+/* Step implementations. */
+enum { RS_MODE_PLAIN = 0, RS_MODE_FUSED = 1 };
+
+/* Stages of one substep, in execution order (see rs_stage). */
+enum {
+    RS_STAGE_FK = 0,          /* forward kinematics, COM, geom poses, velocities (func_update_cartesian_space +
+                                 func_forward_velocity, forward_kinematics.py:1622-1698, :1172-1214) */
+    RS_STAGE_CRBA = 1,        /* mass matrix (forward_dynamics.py:292-541) */
+    RS_STAGE_FACTOR = 2,      /* L^T D L factorisation (forward_dynamics.py:545-684) */
+    RS_STAGE_RNE = 3,         /* actuation, passive, bias forces, acc_smooth (forward_dynamics.py:956-1563) */
+    RS_STAGE_AABB = 4,        /* world AABBs (forward_kinematics.py:1297-1320) */
+    RS_STAGE_BROADPHASE = 5,  /* sweep-and-prune (collider/broadphase.py:141-396) */
+    RS_STAGE_NARROWPHASE = 6, /* contacts (collider/narrowphase.py:1988-2189) */
+    RS_STAGE_CONSTRAINTS = 7, /* equality, friction-loss, contact and limit rows (constraint/solver.py:580-1286) */
+    RS_STAGE_SOLVE = 8,       /* Newton solve + qacc (constraint/solver.py:2936-3261) */
+    RS_STAGE_INTEGRATE = 9,   /* cacc, semi-implicit Euler, NaN guard (rigid_solver.py:2783-2891) */
+    RS_STAGE_COUNT = 10,
+};
+
+/* ------------------------------------------------------------------------------------------------ model blob */
+
+/* Element count kinds used by the model-array and state-field directories. */
+enum {
+    RS_N_ONE = 0, RS_N_LINKS, RS_N_JOINTS, RS_N_DOFS, RS_N_QS, RS_N_GEOMS, RS_N_ENTITIES, RS_N_EQUALITIES,
+    RS_N_DOFS2, RS_N_GEOMS2, RS_N_PAIRS, RS_N_SUPPORT, RS_N_CONTACTS, RS_N_CONSTRAINTS, RS_N_CONSTRAINTS_DOFS,
+    RS_N_BROAD, RS_N_SORT, RS_N_KINDS
+};
+
+#define RS_MAX_MODEL_ARRAYS 96
+
+typedef struct rs_model_header {
+    int32_t abi_version;     /* RS_ABI_VERSION */
+    int32_t float_bytes;     /* 4 for the device blob */
+    int32_t n_links, n_joints, n_dofs, n_qs, n_geoms, n_entities, n_equalities;
+    int32_t n_pairs, n_support;
+    int32_t max_collision_pairs, max_contact_pairs, max_collision_pairs_broad, n_contacts_per_pair;
+    int32_t len_constraints;
+    /* options (RigidOptions / SimOptions subset, genesis/options/solvers.py:482-535) */
+    float dt;
+    float gravity[3];
+    int32_t iterations;
+    float tolerance;
+    int32_t ls_iterations;
+    float ls_tolerance;
+    int32_t enable_collision, enable_joint_limit, disable_constraint, enable_multi_contact, box_box_detection;
+    int32_t integrator, solver_type;
+    float meaninertia;
+    float mc_perturbation, mc_tolerance, mpr_to_gjk_overlap_ratio;
+    int32_t n_arrays;                                /* number of directory entries used */
+    int64_t array_offset[RS_MAX_MODEL_ARRAYS];       /* byte offset of each array in the blob */
+} rs_model_header;
+
+typedef struct rs_array_info {
+    const char* name;
+    int32_t is_int;   /* 1 = int32, 0 = float */
+    int32_t count;    /* RS_N_* kind */
+    int32_t width;    /* scalars per element */
+} rs_array_info;
+
+/* Directory of model arrays (name, type, element count kind, width) in blob order. */
+int rs_model_array_count(void);
+int rs_model_array_info(int index, rs_array_info* out);
+
+typedef struct rs_model rs_model;
+
+/* Upload a model blob (layout per the directory above) to the current device. */
+int rs_model_create(const rs_model_header* header, const void* blob, size_t blob_bytes, rs_model** out);
+void rs_model_destroy(rs_model* model);
+
+/* ------------------------------------------------------------------------------------------------ state slab */
+
+/* Directory of per-environment state fields: each is a [rows][n_envs] plane of float32/int32. */
+int rs_state_field_count(void);
+int rs_state_field_info(int index, rs_array_info* out);
+
+/* Byte offset of every state field for n_envs environments, and the slab size. offsets has rs_state_field_count()
+ * entries. */
+int rs_state_layout(const rs_model* model, int n_envs, int64_t* offsets, int64_t* total_bytes);
+
+/* Initialise a slab: qpos = qpos0, zero velocities, FORCE control mode, cleared caches, then forward kinematics
+ * (Scene.build + reset, scene.py:841-905). */
+int rs_state_init(const rs_model* model, void* state, int n_envs, void* stream);
+
+/* Run n_substeps substeps of RigidSolver.substep on every environment. mode: RS_MODE_PLAIN (one kernel per stage,
+ * state round-trips through the slab) or RS_MODE_FUSED (one persistent kernel per substep). */
+int rs_step(const rs_model* model, void* state, int n_envs, int n_substeps, int mode, void* stream);
+
+/* Run one stage of the plain path (RS_STAGE_*), for per-stage parity checks. */
+int rs_stage(const rs_model* model, void* state, int n_envs, int stage, void* stream);
+
+/* Forward kinematics + velocities only on the environments whose mask entry is non-zero (device int32[n_envs]);
+ * used after set_qpos / set_dofs_position (rigid_solver.py:2155-2199). mask may be NULL for all environments. */
+int rs_forward_kinematics(const rs_model* model, void* state, int n_envs, const int32_t* env_mask, void* stream);
+
+/* Reset warm-start caches (constraint qacc_ws/is_warmstart, collider normal cache, SAP buffer) of masked envs. */
+int rs_reset_caches(const rs_model* model, void* state, int n_envs, const int32_t* env_mask, void* stream);
+
+/* Bitwise OR of the per-environment error words into *out (synchronises the stream). */
+int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_t* out, void* stream);
+
+/* hipGraph capture of n_substeps fused substeps on a stream; replay with rs_graph_launch. */
+typedef struct rs_graph rs_graph;
+int rs_graph_create(const rs_model* model, void* state, int n_envs, int n_substeps, int mode, void* stream,
+                    rs_graph** out);
+int rs_graph_launch(rs_graph* graph, void* stream);
+void rs_graph_destroy(rs_graph* graph);
+
+const char* rs_last_error(void);
+int rs_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIGIDSTEP_H */

@@ -1,0 +1,54 @@
+"""The C-ABI library loads on a host without a GPU and exports what include/rigidstep.h declares."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from genesis_amd import runtime
+from genesis_amd.model.blob import ModelHeader, model_array_directory, pack_model
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(REPO, "include", "rigidstep.h")).read()
+    return sorted(set(re.findall(r"\b(rs_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.load_library()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_model_directory_matches_library():
+    parsed = [(n, i, k.replace("RS_N_", ""), w) for n, i, k, w in model_array_directory()]
+    assert parsed == runtime.model_array_directory()
+
+
+def test_header_struct_size_matches_c():
+    # 16 int32 + float dt + 3 floats + ... : recompute from the ctypes layout and the C header
+    assert ctypes.sizeof(ModelHeader) % 8 == 0
+    assert ModelHeader.array_offset.offset % 8 == 0
+
+
+def test_state_directory_contains_persistent_fields():
+    names = [n for n, *_ in runtime.state_field_directory()]
+    for n in ("qpos", "vel", "acc", "ctrl_mode", "ctrl_pos", "ctrl_vel", "ctrl_force", "qacc_ws", "errbits"):
+        assert n in names
+
+
+def test_pack_model_offsets_are_aligned_and_in_bounds():
+    from genesis_amd.scenes import franka_model
+
+    m = franka_model()
+    h, blob = pack_model(m, np.float32)
+    for i in range(h.n_arrays):
+        assert h.array_offset[i] % 64 == 0
+        assert h.array_offset[i] <= len(blob)
