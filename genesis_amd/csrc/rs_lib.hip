@@ -146,7 +146,9 @@ __global__ void __launch_bounds__(kBlock) k_reset_caches(DModel m, DState st, co
     SF(errbits, 0) = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_stage(DModel m, DState st, int stage) {
+// one kernel per stage (distinct symbols, so kernel traces attribute time per stage)
+template <int stage>
+__global__ void __launch_bounds__(kBlock) k_stage(DModel m, DState st) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= st.B) return;
     switch (stage) {
@@ -200,7 +202,19 @@ static int launch_stage(const rs_model* m, const DState& st, int stage, hipStrea
         // no candidate pairs: detection stops after the AABBs (collider.py:690-691)
     } else if ((stage == RS_STAGE_CONSTRAINTS || stage == RS_STAGE_SOLVE) && dm.disable_constraint) {
     } else {
-        hipLaunchKernelGGL(k_stage, env_grid(st.B), dim3(kBlock), 0, s, dm, st, stage);
+        dim3 g = env_grid(st.B), blk(kBlock);
+        switch (stage) {
+            case RS_STAGE_FK: hipLaunchKernelGGL(k_stage<RS_STAGE_FK>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_CRBA: hipLaunchKernelGGL(k_stage<RS_STAGE_CRBA>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_FACTOR: hipLaunchKernelGGL(k_stage<RS_STAGE_FACTOR>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_RNE: hipLaunchKernelGGL(k_stage<RS_STAGE_RNE>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_BROADPHASE: hipLaunchKernelGGL(k_stage<RS_STAGE_BROADPHASE>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_NARROWPHASE: hipLaunchKernelGGL(k_stage<RS_STAGE_NARROWPHASE>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_CONSTRAINTS: hipLaunchKernelGGL(k_stage<RS_STAGE_CONSTRAINTS>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_SOLVE: hipLaunchKernelGGL(k_stage<RS_STAGE_SOLVE>, g, blk, 0, s, dm, st); break;
+            case RS_STAGE_INTEGRATE: hipLaunchKernelGGL(k_stage<RS_STAGE_INTEGRATE>, g, blk, 0, s, dm, st); break;
+            default: break;
+        }
     }
     HIP_OK(hipGetLastError());
     return RS_OK;
