@@ -19,6 +19,9 @@ FLAGS = [
     "-shared",
     # packed FP32 instructions are disabled on purpose (wrong results have been observed on this hardware)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no FMA contraction: every product is rounded before it is added, as in the reference's sequential sums, so that
+    # the per-stage and fused kernels (whose loops are shaped differently) compute bit-identical values
+    "-ffp-contract=off",
 ]
 
 

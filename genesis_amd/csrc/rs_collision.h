@@ -21,17 +21,20 @@ __device__ inline bool aabb_overlap(const DState& st, int b, int ga, int gb) {
     return true;
 }
 
-// broadphase.py:74-138 + :141-396 _func_broad_phase_sap
-__device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) {
-    int nc = SF(n_contacts, 0);
-    for (int i = 0; i < nc; i++) {
+// broadphase.py:74-138 func_collision_clear: reset the contact rows [from, to) of the previous substep
+__device__ inline void clear_contacts(const DState& st, int b, int from, int to) {
+    for (int i = from; i < to; i++) {
         SF(contact_link_a, i) = SF(contact_link_b, i) = SF(contact_geom_a, i) = SF(contact_geom_b, i) = -1;
         SF(contact_penetration, i) = 0.f;
         ST3(contact_pos, i, zero3());
         ST3(contact_normal, i, zero3());
         ST3(contact_force, i, zero3());
     }
-    SF(n_contacts, 0) = 0;
+}
+
+// broadphase.py:141-396 _func_broad_phase_sap. (st, b) holds the AABBs, sort buffers, active list and pair list;
+// (gs, ge) holds the warm-start normal cache (the same view in the per-stage kernels).
+__device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, const DState& gs, int ge) {
     const int G = m.G;
     if (SF(first_time, 0)) {
         int ib = 0;
@@ -80,7 +83,7 @@ __device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) 
                 int pidx = m.collision_pair_idx[ga * G + gb];
                 if (pidx == -1) continue;
                 if (!aabb_overlap(st, b, ga, gb)) {
-                    ST3(cache_normal, pidx, zero3());
+                    st3(gs.cache_normal, pidx, gs.B, ge, zero3());
                     continue;
                 }
                 if (n_broad == m.BROAD) {
@@ -104,6 +107,12 @@ __device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) 
         }
     }
     SF(n_broad, 0) = n_broad;
+}
+
+__device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) {
+    clear_contacts(st, b, 0, SF(n_contacts, 0));
+    SF(n_contacts, 0) = 0;
+    broadphase_sap(m, st, b, st, b);
 }
 
 // ----------------------------------------------------------------------------------------------- support
@@ -378,28 +387,69 @@ __device__ inline float geom_tolerance(const DModel& m, int ga, int gb, float to
     return 0.5f * tol * s;
 }
 
-// contact.py:181-222 func_add_contact
+// contact.py:181-222 func_add_contact: write contact row i of (st, b)
+__device__ inline void write_contact(const DModel& m, const DState& st, int b, int i, int ga, int gb, f3 n, f3 p,
+                                     float pen, int pair) {
+    float fa = m.geom_friction[ga] * 1.f, fb = m.geom_friction[gb] * 1.f;
+    SF(contact_geom_a, i) = ga;
+    SF(contact_geom_b, i) = gb;
+    ST3(contact_normal, i, n);
+    ST3(contact_pos, i, p);
+    SF(contact_penetration, i) = pen;
+    SF(contact_friction, i) = fmaxf(fmaxf(fa, fb), 1e-2f);
+    for (int k = 0; k < 7; k++)
+        SF(contact_sol_params, 7 * i + k) = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
+    SF(contact_link_a, i) = m.geom_link[ga];
+    SF(contact_link_b, i) = m.geom_link[gb];
+    SF(contact_pair, i) = pair;
+}
+
 __device__ inline void add_contact(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3 p, float pen,
                                    int pair) {
     int i = SF(n_contacts, 0);
     if (i < m.KMAX) {
-        float fa = m.geom_friction[ga] * 1.f, fb = m.geom_friction[gb] * 1.f;
-        SF(contact_geom_a, i) = ga;
-        SF(contact_geom_b, i) = gb;
-        ST3(contact_normal, i, n);
-        ST3(contact_pos, i, p);
-        SF(contact_penetration, i) = pen;
-        SF(contact_friction, i) = fmaxf(fmaxf(fa, fb), 1e-2f);
-        for (int k = 0; k < 7; k++)
-            SF(contact_sol_params, 7 * i + k) = 0.5f * (m.geom_sol_params[7 * ga + k] + m.geom_sol_params[7 * gb + k]);
-        SF(contact_link_a, i) = m.geom_link[ga];
-        SF(contact_link_b, i) = m.geom_link[gb];
-        SF(contact_pair, i) = pair;
+        write_contact(m, st, b, i, ga, gb, n, p, pen, pair);
         SF(n_contacts, 0) = i + 1;
     } else {
         SF(errbits, 0) = SF(errbits, 0) | RS_ERR_OVERFLOW_COLLISION_PAIRS;
     }
 }
+
+// Contact sinks used by the pair routines. PlaneSink appends to the contact rows of one environment (the sequential
+// func_add_contact); SlotSink keeps one pair's contacts in a private slot array (8 floats per contact: normal, pos,
+// penetration) so that pairs can run on different lanes and be compacted in pair order afterwards.
+struct PlaneSink {
+    const DModel& m;
+    const DState& st;
+    int b;
+    __device__ void add(int ga, int gb, f3 n, f3 p, float pen, int pair) { add_contact(m, st, b, ga, gb, n, p, pen, pair); }
+    // true when contact (count - 1 - back) exists and lies within tol of p
+    __device__ bool near(int back, f3 p, float tol) const {
+        int ip = SF(n_contacts, 0) - 1 - back;
+        return ip >= 0 && norm3(p - LD3(contact_pos, ip)) < tol;
+    }
+};
+
+struct SlotSink {
+    float* slot;  // capacity: n_contacts_per_pair entries of 8 floats
+    int cap;
+    int n;
+    __device__ void add(int, int, f3 nrm, f3 p, float pen, int) {
+        if (n < cap) {
+            float* s = slot + 8 * n;
+            s[0] = nrm.x; s[1] = nrm.y; s[2] = nrm.z;
+            s[3] = p.x; s[4] = p.y; s[5] = p.z;
+            s[6] = pen;
+            n++;
+        }
+    }
+    __device__ bool near(int back, f3 p, float tol) const {
+        int ip = n - 1 - back;
+        if (ip < 0) return false;
+        const float* s = slot + 8 * ip;
+        return norm3(p - mk3(s[3], s[4], s[5])) < tol;
+    }
+};
 
 // contact.py:354-412 (principal inertia axes of the link of the smaller geom)
 __device__ inline void contact_axes(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3& a0, f3& a1) {
@@ -522,8 +572,11 @@ __device__ inline bool sphere_capsule(const DModel& m, int ga, int gb, f3 pa, f4
     return col;
 }
 
-// narrowphase.py:520-970 func_convex_convex_contact (MPR code path)
-__device__ inline void convex_convex(const DModel& m, const DState& st, int b, int ga, int gb) {
+// narrowphase.py:520-970 func_convex_convex_contact (MPR code path). Geometry is read from (st, b); the warm-start
+// normal cache lives in (gs, ge); contacts go to the sink.
+template <class Sink>
+__device__ inline void convex_convex(const DModel& m, const DState& st, int b, const DState& gs, int ge, int ga, int gb,
+                                     Sink& sink) {
     int ta = m.geom_type[ga], tb = m.geom_type[gb];
     if (ta == GT_PLANE && tb == GT_BOX) return;
     const float eps = m.eps, mcp = m.mc_perturbation;
@@ -560,7 +613,7 @@ __device__ inline void convex_convex(const DModel& m, const DState& st, int b, i
                 col = pen > 0.f;
             } else {
                 bool updated = false;
-                f3 nws = LD3(cache_normal, pair);
+                f3 nws = ld3(gs.cache_normal, pair, gs.B, ge);
                 bool avail = fabsf(nws.x) > eps || fabsf(nws.y) > eps || fabsf(nws.z) > eps;
                 for (int i_mpr = 0; i_mpr < 2; i_mpr++) {
                     if (i_mpr == 1 && det == 0 && !col && avail) {
@@ -581,14 +634,14 @@ __device__ inline void convex_convex(const DModel& m, const DState& st, int b, i
             pen0 = pen;
             p0 = cp;
             if (col0) {
-                add_contact(m, st, b, ga, gb, n0, p0, pen0, pair);
+                sink.add(ga, gb, n0, p0, pen0, pair);
                 if (multi) {
                     contact_axes(m, st, b, ga, gb, n, ax0, ax1);
                     n_con = 1;
                 }
-                ST3(cache_normal, pair, n);
+                st3(gs.cache_normal, pair, gs.B, ge, n);
             } else {
-                ST3(cache_normal, pair, zero3());
+                st3(gs.cache_normal, pair, gs.B, ge, zero3());
             }
         } else if (multi && col) {
             f3 cpa = qrot((cp - 0.5f * pen * n) - p0, qinv(qr)) + p0;
@@ -599,14 +652,11 @@ __device__ inline void convex_convex(const DModel& m, const DState& st, int b, i
             n = n + cross3(tw, n);
             pen = dot3(n, cpb - cpa);
             bool rep = false;
-            int nct = SF(n_contacts, 0);
-            for (int i = 0; i < n_con; i++) {
-                int ip = nct - 1 - i;
-                if (!rep && ip >= 0 && norm3(cp - LD3(contact_pos, ip)) < tolerance) rep = true;
-            }
+            for (int i = 0; i < n_con; i++)
+                if (!rep && sink.near(i, cp, tolerance)) rep = true;
             if (!rep && pen > -tolerance) {
                 pen = fmaxf(pen, 0.f);
-                add_contact(m, st, b, ga, gb, n, cp, pen, pair);
+                sink.add(ga, gb, n, cp, pen, pair);
                 n_con++;
             }
         }
@@ -614,7 +664,8 @@ __device__ inline void convex_convex(const DModel& m, const DState& st, int b, i
 }
 
 // box_contact.py:25-97 func_plane_box_contact
-__device__ inline void plane_box(const DModel& m, const DState& st, int b, int ga, int gb) {
+template <class Sink>
+__device__ inline void plane_box(const DModel& m, const DState& st, int b, int ga, int gb, Sink& sink) {
     f3 pa = LD3(geoms_pos, ga), pb = LD3(geoms_pos, gb);
     f4q qa = LD4(geoms_quat, ga), qb = LD4(geoms_quat, gb);
     int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
@@ -623,7 +674,7 @@ __device__ inline void plane_box(const DModel& m, const DState& st, int b, int g
     float pen = dot3(n, v1 - pa);
     if (!(pen > 0.f)) return;
     f3 cp = v1 - 0.5f * pen * n;
-    add_contact(m, st, b, ga, gb, n, cp, pen, pair);
+    sink.add(ga, gb, n, cp, pen, pair);
     if (!m.enable_multi_contact) return;
     int n_con = 1;
     float tol = geom_tolerance(m, ga, gb, m.mc_tolerance);
@@ -636,7 +687,7 @@ __device__ inline void plane_box(const DModel& m, const DState& st, int b, int g
         if (p > 0.f) {
             f3 c2 = corner - 0.5f * p * n;
             if (norm3(c2 - cp) > tol) {
-                add_contact(m, st, b, ga, gb, n, c2, p, pair);
+                sink.add(ga, gb, n, c2, p, pair);
                 n_con++;
             }
         }
@@ -644,7 +695,13 @@ __device__ inline void plane_box(const DModel& m, const DState& st, int b, int g
 }
 
 // narrowphase.py:1988-2060 + :2135-2189
+__device__ inline bool convex_pair(const DModel& m, int ta, int tb, int ga, int gb) {
+    return m.geom_is_convex[ga] && m.geom_is_convex[gb] && tb != GT_TERRAIN &&
+           !(m.box_box_detection && ta == GT_BOX && tb == GT_BOX) && !(ta == GT_PLANE && tb == GT_BOX);
+}
+
 __device__ inline void dev_narrowphase(const DModel& m, const DState& st, int b) {
+    PlaneSink sink{m, st, b};
     int nb = SF(n_broad, 0);
     for (int i = 0; i < nb; i++) {
         int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);
@@ -654,9 +711,7 @@ __device__ inline void dev_narrowphase(const DModel& m, const DState& st, int b)
             gb = t;
         }
         int ta = m.geom_type[ga], tb = m.geom_type[gb];
-        if (m.geom_is_convex[ga] && m.geom_is_convex[gb] && tb != GT_TERRAIN &&
-            !(m.box_box_detection && ta == GT_BOX && tb == GT_BOX) && !(ta == GT_PLANE && tb == GT_BOX))
-            convex_convex(m, st, b, ga, gb);
+        if (convex_pair(m, ta, tb, ga, gb)) convex_convex(m, st, b, st, b, ga, gb, sink);
     }
     for (int i = 0; i < nb; i++) {
         int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);
@@ -665,7 +720,7 @@ __device__ inline void dev_narrowphase(const DModel& m, const DState& st, int b)
             ga = gb;
             gb = t;
         }
-        if (m.geom_type[ga] == GT_PLANE && m.geom_type[gb] == GT_BOX) plane_box(m, st, b, ga, gb);
+        if (m.geom_type[ga] == GT_PLANE && m.geom_type[gb] == GT_BOX) plane_box(m, st, b, ga, gb, sink);
     }
 }
 

@@ -8,10 +8,24 @@ namespace rs {
 
 #define JAC(c, d) st.jac[((size_t)(c) * m.D + (d)) * st.B + b]
 
-// solver.py:801-878 func_equality_joint via :881-938
-__device__ inline void dev_equality_constraints(const DModel& m, const DState& st, int b) {
+// Row store of the per-stage kernels: the [row][env] planes of the slab.
+struct PlaneRows {
+    const DState& st;
+    int b, D;
+    __device__ __forceinline__ float& jac(int c, int d) const { return st.jac[((size_t)c * D + d) * st.B + b]; }
+    __device__ __forceinline__ float& diag(int c) const { return st.diag[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& aref(int c) const { return st.aref[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& efc_D(int c) const { return st.efc_D[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& fl(int c) const { return st.efc_frictionloss[(size_t)c * st.B + b]; }
+};
+
+// Rows are built from the kinematic state (st, b); contact records are read from (cs, cb).
+
+// solver.py:801-878 func_equality_joint via :881-938; returns the number of rows written from row 0
+template <class R>
+__device__ inline int equality_rows(const DModel& m, const DState& st, int b, const R& rows) {
     const int D = m.D;
-    int nc = 0, neq = 0;
+    int nc = 0;
     for (int e = 0; e < m.NE; e++) {
         if (m.eq_type[e] != EQ_JOINT) continue;
         const float* sp = &m.eq_sol_params[7 * e];
@@ -20,8 +34,7 @@ __device__ inline void dev_equality_constraints(const DModel& m, const DState& s
         int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
         int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
         int c = nc++;
-        neq++;
-        for (int d = 0; d < D; d++) JAC(c, d) = 0.f;
+        for (int d = 0; d < D; d++) rows.jac(c, d) = 0.f;
         float diff = SF(qpos, q2) - m.qpos0[q2];
         float pos = SF(qpos, q1) - m.qpos0[q1];
         float deriv = 0.f, dp = 1.f;
@@ -30,25 +43,24 @@ __device__ inline void dev_equality_constraints(const DModel& m, const DState& s
             if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * (float)(i5 + 1);
             dp = dp * diff;
         }
-        JAC(c, d1) = 1.f;
-        JAC(c, d2) = -deriv;
-        float jq = JAC(c, d1) * SF(vel, d1) + JAC(c, d2) * SF(vel, d2);
+        rows.jac(c, d1) = 1.f;
+        rows.jac(c, d2) = -deriv;
+        float jq = rows.jac(c, d1) * SF(vel, d1) + rows.jac(c, d2) * SF(vel, d2);
         float invw = m.dof_invweight[d1] + m.dof_invweight[d2];
         float imp, aref;
         imp_aref(sp, -fabsf(pos), jq, pos, imp, aref);
         float dg = fmaxf(invw * (1.f - imp) / imp, m.eps);
-        SF(diag, c) = dg;
-        SF(aref, c) = aref;
-        SF(efc_D, c) = 1.f / dg;
+        rows.diag(c) = dg;
+        rows.aref(c) = aref;
+        rows.efc_D(c) = 1.f / dg;
     }
-    SF(n_con, 0) = nc;
-    SF(n_con_eq, 0) = neq;
+    return nc;
 }
 
-// solver.py:1237-1286, :580-685, :1177-1233
-__device__ inline void dev_inequality_constraints(const DModel& m, const DState& st, int b) {
+// solver.py:1237-1286 add_frictionloss_constraints; appends from row nc, returns the number of rows added
+template <class R>
+__device__ inline int frictionloss_rows(const DModel& m, const DState& st, int b, const R& rows, int nc) {
     const int D = m.D;
-    int nc = SF(n_con, 0);
     int nfl = 0;
     for (int l = 0; l < m.L; l++)
         for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++)
@@ -57,85 +69,110 @@ __device__ inline void dev_inequality_constraints(const DModel& m, const DState&
                 float imp, aref;
                 imp_aref(&m.joint_sol_params[7 * j], 0.f, 1.f * SF(vel, d), 0.f, imp, aref);
                 float dg = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
-                int c = nc++;
-                nfl++;
-                SF(diag, c) = dg;
-                SF(aref, c) = aref;
-                SF(efc_D, c) = 1.f / dg;
-                SF(efc_frictionloss, c) = m.dof_frictionloss[d];
-                for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
-                JAC(c, d) = 1.f;
+                int c = nc + nfl++;
+                rows.diag(c) = dg;
+                rows.aref(c) = aref;
+                rows.efc_D(c) = 1.f / dg;
+                rows.fl(c) = m.dof_frictionloss[d];
+                for (int k = 0; k < D; k++) rows.jac(c, k) = 0.f;
+                rows.jac(c, d) = 1.f;
             }
+    return nfl;
+}
+
+// solver.py:580-685 add_collision_constraints for contact ic: rows start + 4 ic .. + 3
+template <class R>
+__device__ inline void contact_rows(const DModel& m, const DState& st, int b, const DState& cs, int cb, const R& rows,
+                                    int ic, int start) {
+    const int D = m.D;
+    const size_t CB = (size_t)cs.B;
+    int la = cs.contact_link_a[ic * CB + cb], lb = cs.contact_link_b[ic * CB + cb];
+    f3 cpos = ld3(cs.contact_pos, ic, cs.B, cb), nrm = ld3(cs.contact_normal, ic, cs.B, cb);
+    float fr = cs.contact_friction[ic * CB + cb], pen = cs.contact_penetration[ic * CB + cb];
+    float sp[7];
+    for (int k = 0; k < 7; k++) sp[k] = cs.contact_sol_params[(size_t)(7 * ic + k) * CB + cb];
+    f3 d1, d2;
+    orthogonals(nrm, d1, d2);
+    float invw = m.link_invweight[2 * la];
+    if (lb > -1) invw = invw + m.link_invweight[2 * lb];
+    for (int i = 0; i < 4; i++) {
+        f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
+        f3 n = dd * fr - nrm;
+        int c = start + ic * 4 + i;
+        for (int k = 0; k < D; k++) rows.jac(c, k) = 0.f;
+        float jq = 0.f;
+        for (int ab = 0; ab < 2; ab++) {
+            float sign = ab == 0 ? -1.f : 1.f;
+            int link = ab == 0 ? la : lb;
+            while (link > -1) {
+                f3 tpos = cpos - LD3(root_COM, link);
+                for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
+                    f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
+                    f3 vel = qrot(cv - cross3(tpos, ca), qident());
+                    float jj = dot3(sign * vel, n);
+                    jq = jq + jj * SF(vel, k);
+                    rows.jac(c, k) = rows.jac(c, k) + jj;
+                }
+                link = m.link_parent[link];
+            }
+        }
+        float imp, aref;
+        imp_aref(sp, -pen, jq, -pen, imp, aref);
+        float dg = invw + fr * fr * invw;
+        dg *= 2.f * fr * fr * (1.f - imp) / imp;
+        dg = fmaxf(dg, m.eps);
+        rows.diag(c) = dg;
+        rows.aref(c) = aref;
+        rows.efc_D(c) = 1.f / dg;
+    }
+}
+
+// solver.py:1177-1233 add_joint_limit_constraints; appends from row nc, returns the new row count
+template <class R>
+__device__ inline int limit_rows(const DModel& m, const DState& st, int b, const R& rows, int nc) {
+    const int D = m.D;
+    for (int l = 0; l < m.L; l++)
+        for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
+            int jt = m.joint_type[j];
+            if (jt != JT_REVOLUTE && jt != JT_PRISMATIC) continue;
+            int q = m.joint_q_start[j], d = m.joint_dof_start[j];
+            float dmin = SF(qpos, q) - m.dof_limit[2 * d];
+            float dmax = m.dof_limit[2 * d + 1] - SF(qpos, q);
+            float pd = fminf(dmin, dmax);
+            if (pd < 0.f) {
+                float jj = (float)((dmin < dmax) ? 1 : 0) * 2.f - 1.f;
+                float imp, aref;
+                imp_aref(&m.joint_sol_params[7 * j], pd, jj * SF(vel, d), pd, imp, aref);
+                float dg = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
+                int c = nc++;
+                rows.diag(c) = dg;
+                rows.aref(c) = aref;
+                rows.efc_D(c) = 1.f / dg;
+                for (int k = 0; k < D; k++) rows.jac(c, k) = 0.f;
+                rows.jac(c, d) = jj;
+            }
+        }
+    return nc;
+}
+
+__device__ inline void dev_equality_constraints(const DModel& m, const DState& st, int b) {
+    int nc = equality_rows(m, st, b, PlaneRows{st, b, m.D});
+    SF(n_con, 0) = nc;
+    SF(n_con_eq, 0) = nc;
+}
+
+__device__ inline void dev_inequality_constraints(const DModel& m, const DState& st, int b) {
+    PlaneRows rows{st, b, m.D};
+    int nc = SF(n_con, 0);
+    int nfl = frictionloss_rows(m, st, b, rows, nc);
+    nc += nfl;
     SF(n_con_fl, 0) = nfl;
     if (m.enable_collision) {
         int ncont = SF(n_contacts, 0);
-        int start = nc;
-        for (int ic = 0; ic < ncont; ic++) {
-            int la = SF(contact_link_a, ic), lb = SF(contact_link_b, ic);
-            f3 cpos = LD3(contact_pos, ic), nrm = LD3(contact_normal, ic);
-            float fr = SF(contact_friction, ic), pen = SF(contact_penetration, ic);
-            float sp[7];
-            for (int k = 0; k < 7; k++) sp[k] = SF(contact_sol_params, 7 * ic + k);
-            f3 d1, d2;
-            orthogonals(nrm, d1, d2);
-            float invw = m.link_invweight[2 * la];
-            if (lb > -1) invw = invw + m.link_invweight[2 * lb];
-            for (int i = 0; i < 4; i++) {
-                f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
-                f3 n = dd * fr - nrm;
-                int c = start + ic * 4 + i;
-                for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
-                float jq = 0.f;
-                for (int ab = 0; ab < 2; ab++) {
-                    float sign = ab == 0 ? -1.f : 1.f;
-                    int link = ab == 0 ? la : lb;
-                    while (link > -1) {
-                        f3 tpos = cpos - LD3(root_COM, link);
-                        for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
-                            f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
-                            f3 vel = qrot(cv - cross3(tpos, ca), qident());
-                            float jj = dot3(sign * vel, n);
-                            jq = jq + jj * SF(vel, k);
-                            JAC(c, k) = JAC(c, k) + jj;
-                        }
-                        link = m.link_parent[link];
-                    }
-                }
-                float imp, aref;
-                imp_aref(sp, -pen, jq, -pen, imp, aref);
-                float dg = invw + fr * fr * invw;
-                dg *= 2.f * fr * fr * (1.f - imp) / imp;
-                dg = fmaxf(dg, m.eps);
-                SF(diag, c) = dg;
-                SF(aref, c) = aref;
-                SF(efc_D, c) = 1.f / dg;
-            }
-        }
+        for (int ic = 0; ic < ncont; ic++) contact_rows(m, st, b, st, b, rows, ic, nc);
         nc = nc + ncont * 4;
     }
-    if (m.enable_joint_limit) {
-        for (int l = 0; l < m.L; l++)
-            for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
-                int jt = m.joint_type[j];
-                if (jt != JT_REVOLUTE && jt != JT_PRISMATIC) continue;
-                int q = m.joint_q_start[j], d = m.joint_dof_start[j];
-                float dmin = SF(qpos, q) - m.dof_limit[2 * d];
-                float dmax = m.dof_limit[2 * d + 1] - SF(qpos, q);
-                float pd = fminf(dmin, dmax);
-                if (pd < 0.f) {
-                    float jj = (float)((dmin < dmax) ? 1 : 0) * 2.f - 1.f;
-                    float imp, aref;
-                    imp_aref(&m.joint_sol_params[7 * j], pd, jj * SF(vel, d), pd, imp, aref);
-                    float dg = fmaxf(m.dof_invweight[d] * (1.f - imp) / imp, m.eps);
-                    int c = nc++;
-                    SF(diag, c) = dg;
-                    SF(aref, c) = aref;
-                    SF(efc_D, c) = 1.f / dg;
-                    for (int k = 0; k < D; k++) JAC(c, k) = 0.f;
-                    JAC(c, d) = jj;
-                }
-            }
-    }
+    if (m.enable_joint_limit) nc = limit_rows(m, st, b, rows, nc);
     SF(n_con, 0) = nc;
 }
 

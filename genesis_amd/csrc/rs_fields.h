@@ -115,4 +115,6 @@
     RS_SF(improved, 1, RS_N_ONE, 1)                         \
     RS_SF(ls_it, 1, RS_N_ONE, 1)                            \
     RS_SF(ls_result, 1, RS_N_ONE, 1)                        \
-    RS_SF(n_iter, 1, RS_N_ONE, 1)
+    RS_SF(n_iter, 1, RS_N_ONE, 1)                           \
+    /* env-major working set of the fused substep */       \
+    RS_SF(wk, 0, RS_N_WORK, 1)

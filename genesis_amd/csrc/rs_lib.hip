@@ -2,7 +2,7 @@
 //
 // Plain path: one kernel per stage of RigidSolver.substep (rigid_solver.py:912-980), one lane per environment, every
 // intermediate kept in the [row][env] planes of the state slab (the comparison point and drop-in).
-// Fused path: the whole substep in a single kernel per launch (rs_fused.hip).
+// Fused path: the whole substep in a single kernel per launch, one wavefront per environment (rs_fused.h).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -56,12 +56,17 @@ struct rs_model {
     size_t blob_bytes = 0;
     DModel dm;
     int device = 0;
+    FusedArgs fa;       // fused-kernel layout and launch shape
+    int use_lds = 0;    // private regions in LDS
+    int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
 };
 
 struct rs_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+static int64_t work_floats(const rs_model_header& h);
 
 static int64_t count_of(const rs_model_header& h, int kind) {
     switch (kind) {
@@ -82,8 +87,38 @@ static int64_t count_of(const rs_model_header& h, int kind) {
         case RS_N_CONSTRAINTS_DOFS: return (int64_t)h.len_constraints * h.n_dofs;
         case RS_N_BROAD: return h.max_collision_pairs_broad;
         case RS_N_SORT: return 2 * (int64_t)h.n_geoms;
+        case RS_N_WORK: return work_floats(h);
         default: return 0;
     }
+}
+
+// Private-region offsets of the local fields and the env-major work layout of the fused kernel (floats, each
+// sub-array 16-byte aligned, each environment's slice 256-byte aligned).
+static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
+    auto up4 = [](int64_t x) { return (x + 3) / 4 * 4; };
+    int64_t off = 0;
+    for (int i = 0; i < k_n_state_fields && i < kMaxStateFields; i++) {
+        fa->loc_off[i] = -1;
+        if (!is_local_kind(k_state_fields[i].count)) continue;
+        fa->loc_off[i] = (int)off;
+        int64_t rows = count_of(h, k_state_fields[i].count) * k_state_fields[i].width;
+        off += up4(rows < 1 ? 1 : rows);
+    }
+    fa->loc_floats = (int)off;
+    const int64_t C = h.len_constraints > 0 ? h.len_constraints : 1, D = h.n_dofs;
+    fa->off_loc = 0;
+    fa->off_jac = (int)up4(off);
+    fa->off_rowf = (int)(fa->off_jac + up4(C * D));
+    fa->off_tmp = (int)(fa->off_rowf + RW_COUNT * up4(C));
+    fa->off_pc = (int)(fa->off_tmp + kTmpArrays * up4(C));
+    int64_t total = fa->off_pc + (int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8;
+    fa->work_stride = (total + 63) / 64 * 64;
+}
+
+static int64_t work_floats(const rs_model_header& h) {
+    FusedArgs fa;
+    fused_layout(h, &fa);
+    return fa.work_stride;
 }
 
 // ------------------------------------------------------------------------------------------------ state views
@@ -220,6 +255,24 @@ static int launch_stage(const rs_model* m, const DState& st, int stage, hipStrea
     return RS_OK;
 }
 
+static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
+    const int epb = m->fa.epb;
+    const dim3 grid((unsigned)((st.B + epb - 1) / epb)), block((unsigned)(kWave * epb));
+    if (m->use_lds)
+        hipLaunchKernelGGL(k_fused_wave<true>, grid, block, (size_t)epb * m->fa.loc_floats * 4, s, m->dm, st, m->fa);
+    else
+        hipLaunchKernelGGL(k_fused_wave<false>, grid, block, 0, s, m->dm, st, m->fa);
+    HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+static int check_fused_config(const rs_model* m, int epb, int use_lds) {
+    if (epb < 1 || epb > 8) return fail(RS_E_INVALID, "fused config: envs_per_block must be in 1..8");
+    if (use_lds && (int64_t)epb * m->fa.loc_floats * 4 > m->max_lds)
+        return fail(RS_E_UNSUPPORTED, "fused config: private regions do not fit in LDS at this envs_per_block");
+    return RS_OK;
+}
+
 static int plain_substep(const rs_model* m, const DState& st, hipStream_t s) {
     const int order[] = {RS_STAGE_CRBA, RS_STAGE_FACTOR, RS_STAGE_RNE};
     for (int stg : order)
@@ -311,7 +364,39 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
         rs_model_destroy(m);
         return fail(RS_E_UNSUPPORTED, "rs_model_create: only the approximate_implicitfast integrator is built");
     }
+    static_assert(k_n_state_fields <= kMaxStateFields, "FusedArgs::loc_off too small");
+    fused_layout(m->h, &m->fa);
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device) != hipSuccess) max_lds = 0;
+    m->max_lds = max_lds < 65536 ? max_lds : 65536;
+    // default launch shape: two environments per workgroup, private regions in LDS when they fit
+    m->fa.epb = 2;
+    m->use_lds = check_fused_config(m, 2, 1) == RS_OK;
+    g_err.clear();
     *out = m;
+    return RS_OK;
+}
+
+int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
+    if (!m || !out) return fail(RS_E_INVALID, "rs_fused_config_get: null argument");
+    out->envs_per_block = m->fa.epb;
+    out->use_lds = m->use_lds;
+    out->lds_bytes_per_env = m->fa.loc_floats * 4;
+    return RS_OK;
+}
+
+int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
+    if (!m || !c) return fail(RS_E_INVALID, "rs_fused_config_set: null argument");
+    if (int r = check_fused_config(m, c->envs_per_block, c->use_lds != 0)) return r;
+    m->fa.epb = c->envs_per_block;
+    m->use_lds = c->use_lds != 0;
+    return RS_OK;
+}
+
+int rs_state_field_rows(const rs_model* m, int i, int64_t* rows) {
+    if (!m || !rows || i < 0 || i >= k_n_state_fields) return fail(RS_E_INVALID, "rs_state_field_rows: bad argument");
+    int64_t r = count_of(m->h, k_state_fields[i].count) * k_state_fields[i].width;
+    *rows = r < 1 ? 1 : r;
     return RS_OK;
 }
 
@@ -348,8 +433,8 @@ int rs_step(const rs_model* m, void* state, int n_envs, int n_substeps, int mode
     hipStream_t s = (hipStream_t)stream;
     DState st = make_state(m, state, n_envs);
     for (int k = 0; k < n_substeps; k++) {
-        int r = mode == RS_MODE_PLAIN ? plain_substep(m, st, s) : fused_substep(m->dm, st, s);
-        if (r) return r == RS_E_DEVICE ? fail(r, std::string("launch failed: ") + hipGetErrorString(hipGetLastError())) : r;
+        int r = mode == RS_MODE_PLAIN ? plain_substep(m, st, s) : fused_substep(m, st, s);
+        if (r) return r;
     }
     return RS_OK;
 }

@@ -1,0 +1,492 @@
+// Constraint rows and Newton solve of one environment on one wavefront (fused substep).
+// Same algorithm and arithmetic as rs_constraint.h (constraint/solver.py: rows :580-1286, solve :2936-3261, line
+// search :2034-2597): row-wise work is split across lanes, and every reduction over rows or dofs is summed in the
+// reference's sequential order (by one lane per output, or uniformly by all lanes), so results match the per-stage
+// kernels. Rows live in the env-major work field (Rows); the small matrices (mass matrix, Hessian factor) and dof
+// vectors live in the environment's private region (loc).
+#pragma once
+#include "rs_constraint.h"
+#include "rs_wave.h"
+
+namespace rs {
+
+// Row store of the fused kernel: env-major arrays in the work field.
+struct WorkRows {
+    float* J;   // [c * D + d]
+    float* f;   // [k * cap + c], k < RW_COUNT
+    float* tp;  // [k * cap + c], k < kTmpArrays
+    int cap, D;
+    __device__ __forceinline__ float& jac(int c, int d) const { return J[(size_t)c * D + d]; }
+    __device__ __forceinline__ float& diag(int c) const { return f[(size_t)RW_DIAG * cap + c]; }
+    __device__ __forceinline__ float& aref(int c) const { return f[(size_t)RW_AREF * cap + c]; }
+    __device__ __forceinline__ float& efc_D(int c) const { return f[(size_t)RW_EFC_D * cap + c]; }
+    __device__ __forceinline__ float& fl(int c) const { return f[(size_t)RW_FL * cap + c]; }
+    __device__ __forceinline__ float& Jaref(int c) const { return f[(size_t)RW_JAREF * cap + c]; }
+    __device__ __forceinline__ float& jv(int c) const { return f[(size_t)RW_JV * cap + c]; }
+    __device__ __forceinline__ float& force(int c) const { return f[(size_t)RW_FORCE * cap + c]; }
+    __device__ __forceinline__ int& active(int c) const { return ((int*)f)[(size_t)RW_ACTIVE * cap + c]; }
+    __device__ __forceinline__ int& prev(int c) const { return ((int*)f)[(size_t)RW_PREV * cap + c]; }
+    __device__ __forceinline__ float& tmp(int k, int c) const { return tp[(size_t)k * cap + c]; }
+};
+
+#define LF(field, row) loc.field[(row)]
+
+// solver.py:580-1286: equality, friction-loss and limit rows uniformly (few rows), contact rows one lane per contact
+__device__ inline void wave_constraints(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
+                                        int lane) {
+    int nc = equality_rows(m, loc, 0, R);
+    const int neq = nc;
+    int nfl = frictionloss_rows(m, loc, 0, R, nc);
+    nc += nfl;
+    if (m.enable_collision) {
+        const int ncont = LF(n_contacts, 0);
+        for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc);
+        nc += ncont * 4;
+    }
+    if (m.enable_joint_limit) nc = limit_rows(m, loc, 0, R, nc);
+    LF(n_con, 0) = nc;
+    LF(n_con_eq, 0) = neq;
+    LF(n_con_fl, 0) = nfl;
+    wave_sync();
+}
+
+// solver.py:2620-2694 (same arithmetic as dev_update_constraint)
+__device__ inline float wave_update_constraint(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+                                               float cost_in) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    LF(prev_cost, 0) = cost_in;
+    for (int c = lane; c < nc; c += kWave) {
+        R.prev(c) = R.active(c);
+        int act = 1;
+        float ff = 0.f, fc = 0.f;
+        float ja = R.Jaref(c);
+        if (ne <= c && c < nef) {
+            float f = R.fl(c), r = R.diag(c);
+            float rf = r * f;
+            bool ln = ja <= -rf, lp = ja >= rf;
+            act = !(ln || lp);
+            ff = (float)ln * f + (float)lp * -f;
+            fc = (float)ln * f * (-0.5f * rf - ja);
+            fc = fc + (float)lp * f * (-0.5f * rf + ja);
+        } else if (nef <= c) {
+            act = ja < 0.f;
+        }
+        R.active(c) = act;
+        float Dc = R.efc_D(c);
+        R.force(c) = ff + (-ja * Dc * (float)act);
+        R.tmp(0, c) = fc;
+        R.tmp(1, c) = 0.5f * (ja * ja * Dc * (float)act);
+    }
+    wave_sync();
+    for (int d = lane; d < D; d += kWave) {
+        float q = 0.f;
+        for (int c = 0; c < nc; c++) q = q + R.jac(c, d) * R.force(c);
+        LF(qfrc_constraint, d) = q;
+    }
+    // uniform: the cost in the reference's order (friction-loss terms, Gauss terms, quadratic row terms)
+    float cost = 0.f, gauss = 0.f;
+    for (int c = ne; c < nef; c++) cost = cost + R.tmp(0, c);
+    for (int d = 0; d < D; d++) {
+        float v = 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
+        gauss = gauss + v;
+        cost = cost + v;
+    }
+    for (int c = 0; c < nc; c++) cost = cost + R.tmp(1, c);
+    LF(gauss, 0) = gauss;
+    wave_sync();
+    return cost;
+}
+
+// solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
+__device__ inline void wave_hessian_cholesky(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    const int nE = D * (D + 1) / 2;
+    const float eps = m.eps;
+    for (int e = lane; e < nE; e += kWave) {
+        int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+        while ((i + 1) * (i + 2) / 2 <= e) i++;
+        while (i * (i + 1) / 2 > e) i--;
+        const int j = e - i * (i + 1) / 2;
+        float h = 0.f;
+        for (int c = 0; c < nc; c++) {
+            float ji = R.jac(c, i);
+            if (fabsf(ji) > eps) h = h + R.jac(c, j) * ji * R.efc_D(c) * (float)R.active(c);
+        }
+        if (m.dof_entity[i] == m.dof_entity[j]) h = h + LF(mass_mat, i * D + j);
+        LF(nt_H, i * D + j) = h;
+    }
+    wave_sync();
+    for (int i = 0; i < D; i++) {
+        float t = LF(nt_H, i * D + i);
+        for (int j = 0; j < i; j++) t = t - LF(nt_H, i * D + j) * LF(nt_H, i * D + j);
+        float dii = sqrtf(fmaxf(t, eps));
+        float inv = 1.f / dii;
+        for (int j = i + 1 + lane; j < D; j += kWave) {
+            float dt = 0.f;
+            for (int k = 0; k < i; k++) dt = dt + LF(nt_H, j * D + k) * LF(nt_H, i * D + k);
+            LF(nt_H, j * D + i) = (LF(nt_H, j * D + i) - dt) * inv;
+        }
+        wave_sync();
+        LF(nt_H, i * D + i) = dii;
+    }
+    wave_sync();
+}
+
+// solver.py:1782-1822: rank-1 updates of the factor for rows whose active flag flipped, in row order;
+// returns true (uniformly) when an update degenerates
+__device__ inline bool wave_cholesky_incremental(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+        const int cl = c0 + lane;
+        uint64_t flips = wave_ballot(cl < nc && (R.active(cl) ^ R.prev(cl)));
+        while (flips) {
+            const int c = c0 + __builtin_ctzll(flips);
+            flips &= flips - 1;
+            const float sign = R.active(c) ? 1.f : -1.f;
+            const float sq = sqrtf(R.efc_D(c));
+            for (int d = lane; d < D; d += kWave) LF(nt_vec, d) = R.jac(c, d) * sq;
+            wave_sync();
+            for (int k = 0; k < D; k++) {
+                const float vk = LF(nt_vec, k);
+                if (!(fabsf(vk) > m.eps)) continue;
+                const float Lkk = LF(nt_H, k * D + k);
+                const float t = Lkk * Lkk + sign * vk * vk;
+                if (t < m.eps) return true;
+                const float r = sqrtf(t);
+                const float cc = r / Lkk, cinv = 1.f / cc, ss = vk / Lkk;
+                for (int i = k + 1 + lane; i < D; i += kWave) {
+                    float h = (LF(nt_H, i * D + k) + ss * LF(nt_vec, i) * sign) * cinv;
+                    LF(nt_H, i * D + k) = h;
+                    LF(nt_vec, i) = LF(nt_vec, i) * cc - ss * h;
+                }
+                wave_sync();
+                LF(nt_H, k * D + k) = r;
+            }
+        }
+    }
+    wave_sync();
+    return false;
+}
+
+// solver.py:2722-2751 + :1893-1910 (uniform: D is small; the order of every sum is the reference's)
+__device__ inline void wave_gradient(const DModel& m, const DState& loc) {
+    const int D = m.D;
+    for (int d = 0; d < D; d++) LF(grad, d) = LF(Ma, d) - LF(force, d) - LF(qfrc_constraint, d);
+    for (int i = 0; i < D; i++) {
+        float cur = LF(grad, i);
+        for (int j = 0; j < i; j++) cur = cur - LF(nt_H, i * D + j) * LF(Mgrad, j);
+        LF(Mgrad, i) = cur / LF(nt_H, i * D + i);
+    }
+    for (int ii = 0; ii < D; ii++) {
+        int i = D - 1 - ii;
+        float cur = LF(Mgrad, i);
+        for (int j = i + 1; j < D; j++) cur = cur - LF(nt_H, j * D + i) * LF(Mgrad, j);
+        LF(Mgrad, i) = cur / LF(nt_H, i * D + i);
+    }
+}
+
+// solver.py:2034-2152 func_ls_init_and_eval_p0_opt
+__device__ inline LsP wave_ls_init(const DModel& m, const DState& loc, const WorkRows& R, int lane, int& ls_it) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    for (int i = lane; i < D; i += kWave) {
+        const int e = m.dof_entity[i];
+        float mv = 0.f;
+        for (int j = m.entity_dof_start[e]; j < m.entity_dof_end[e]; j++) mv = mv + LF(mass_mat, i * D + j) * LF(search, j);
+        LF(mv, i) = mv;
+    }
+    for (int c = lane; c < nc; c += kWave) {
+        float jv = 0.f;
+        for (int d = 0; d < D; d++) jv = jv + R.jac(c, d) * LF(search, d);
+        R.jv(c) = jv;
+        float ja = R.Jaref(c), Dc = R.efc_D(c);
+        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+        if (c < ne) {
+        } else if (c < nef) {
+            float f = R.fl(c), r = R.diag(c), rf = r * f;
+            bool ln = ja <= -rf, lp = ja >= rf;
+            if (ln || lp) {
+                f0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+                f1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+                f2 = 0.f;
+            }
+        } else {
+            float a = (float)(ja < 0.f);
+            f0 = f0 * a;
+            f1 = f1 * a;
+            f2 = f2 * a;
+        }
+        R.tmp(0, c) = f0;
+        R.tmp(1, c) = f1;
+        R.tmp(2, c) = f2;
+    }
+    wave_sync();
+    float g1 = 0.f, g2 = 0.f;
+    for (int d = 0; d < D; d++) {
+        float s = LF(search, d);
+        g1 = g1 + (s * LF(Ma, d) - s * LF(force, d));
+        g2 = g2 + 0.5f * s * LF(mv, d);
+    }
+    float gauss = LF(gauss, 0);
+    LF(quad_gauss, 0) = gauss;
+    LF(quad_gauss, 1) = g1;
+    LF(quad_gauss, 2) = g2;
+    float t0 = gauss, t1 = g1, t2 = g2, e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    for (int c = 0; c < nc; c++) {
+        float f0 = R.tmp(0, c), f1 = R.tmp(1, c), f2 = R.tmp(2, c);
+        if (c < ne) {
+            e0 = e0 + f0; e1 = e1 + f1; e2 = e2 + f2;
+        }
+        t0 = t0 + f0; t1 = t1 + f1; t2 = t2 + f2;
+    }
+    LF(eq_sum, 0) = e0;
+    LF(eq_sum, 1) = e1;
+    LF(eq_sum, 2) = e2;
+    float h = 2.f * t2;
+    if (h <= 0.f) h = m.eps;
+    ls_it = 1;
+    wave_sync();
+    return {0.f, t0, t1, h};
+}
+
+// solver.py:2155-2355 (n = 1 or 3 candidate steps in one pass over the rows)
+__device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const WorkRows& R, int lane, const float* al,
+                                    int n, LsP* out, int& ls_it) {
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    for (int c = ne + lane; c < nc; c += kWave) {
+        float ja = R.Jaref(c), jv = R.jv(c), Dc = R.efc_D(c);
+        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+        if (c < nef) {
+            float f = R.fl(c), r = R.diag(c), rf = r * f;
+            for (int k = 0; k < n; k++) {
+                float x = ja + al[k] * jv;
+                bool ln = x <= -rf, lp = x >= rf;
+                float a0 = f0, a1 = f1, a2 = f2;
+                if (ln || lp) {
+                    a0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+                    a1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+                    a2 = 0.f;
+                }
+                R.tmp(3 * k, c) = a0;
+                R.tmp(3 * k + 1, c) = a1;
+                R.tmp(3 * k + 2, c) = a2;
+            }
+        } else {
+            for (int k = 0; k < n; k++) {
+                float a = (float)(ja + al[k] * jv < 0.f);
+                R.tmp(3 * k, c) = f0 * a;
+                R.tmp(3 * k + 1, c) = f1 * a;
+                R.tmp(3 * k + 2, c) = f2 * a;
+            }
+        }
+    }
+    wave_sync();
+    const float b0 = LF(quad_gauss, 0) + LF(eq_sum, 0), b1 = LF(quad_gauss, 1) + LF(eq_sum, 1),
+                b2 = LF(quad_gauss, 2) + LF(eq_sum, 2);
+    for (int k = 0; k < n; k++) {
+        float t0 = b0, t1 = b1, t2 = b2;
+        for (int c = ne; c < nc; c++) {
+            t0 = t0 + R.tmp(3 * k, c);
+            t1 = t1 + R.tmp(3 * k + 1, c);
+            t2 = t2 + R.tmp(3 * k + 2, c);
+        }
+        float a = al[k];
+        float h = 2.f * t2;
+        if (h <= 0.f) h = m.eps;
+        out[k] = {a, a * a * t2 + a * t1 + t0, 2.f * a * t2 + t1, h};
+    }
+    ls_it += n;
+    wave_sync();
+}
+
+// solver.py:2427-2597 func_linesearch_batch (uniform control flow)
+__device__ inline float wave_linesearch(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+    const int D = m.D;
+    float sn = 0.f;
+    for (int d = 0; d < D; d++) sn = sn + LF(search, d) * LF(search, d);
+    sn = sqrtf(sn);
+    float gtol = m.tolerance * m.ls_tolerance * sn * (m.meaninertia * (float)max(1, D));
+    LF(gtol, 0) = gtol;
+    int ls_it = 0, res_code = 0;
+    float res = 0.f;
+    if (sn < m.eps) {
+        LF(ls_result, 0) = 1;
+        LF(ls_it, 0) = 0;
+        return 0.f;
+    }
+    LsP p0 = wave_ls_init(m, loc, R, lane, ls_it);
+    LsP p1;
+    float a = p0.alpha - p0.grad / p0.hess;
+    wave_ls_eval(m, loc, R, lane, &a, 1, &p1, ls_it);
+    if (p0.cost < p1.cost) p1 = p0;
+    bool done = false;
+    if (fabsf(p1.grad) < gtol) {
+        res_code = fabsf(p1.alpha) < m.eps ? 2 : 0;
+        res = p1.alpha;
+        done = true;
+    }
+    LsP p2 = p1;
+    int p2update = 0;
+    if (!done) {
+        float dir = (float)((p1.grad < 0.f) ? 1 : 0) * 2.f - 1.f;
+        while (p1.grad * dir <= -gtol && ls_it < m.ls_iterations) {
+            p2 = p1;
+            p2update = 1;
+            float an = p1.alpha - p1.grad / p1.hess;
+            wave_ls_eval(m, loc, R, lane, &an, 1, &p1, ls_it);
+            if (fabsf(p1.grad) < gtol) {
+                res = p1.alpha;
+                done = true;
+                break;
+            }
+        }
+    }
+    if (!done && ls_it >= m.ls_iterations) {
+        res_code = 3;
+        res = p1.alpha;
+        done = true;
+    }
+    if (!done && !p2update) {
+        res_code = 6;
+        res = p1.alpha;
+        done = true;
+    }
+    if (!done) {
+        float al[3] = {p1.alpha - p1.grad / p1.hess, p1.alpha, (p1.alpha + p2.alpha) * 0.5f};
+        while (ls_it < m.ls_iterations) {
+            LsP c[3];
+            wave_ls_eval(m, loc, R, lane, al, 3, c, ls_it);
+            float n1 = al[0], n2 = al[1];
+            bool found = false;
+            float ba = 0.f, bc = 0.f;
+            for (int i = 0; i < 3; i++)
+                if (fabsf(c[i].grad) < gtol && (!found || c[i].cost < bc)) {
+                    ba = c[i].alpha;
+                    bc = c[i].cost;
+                    found = true;
+                }
+            if (found) {
+                res = ba;
+                done = true;
+                break;
+            }
+            int b1 = bracket(p1, c, n1), b2 = bracket(p2, c, n2);
+            if (b1 == 0 && b2 == 0) {
+                res_code = c[2].cost < p0.cost ? 0 : 7;
+                res = al[2];
+                done = true;
+                break;
+            }
+            al[0] = n1;
+            al[1] = n2;
+            al[2] = (p1.alpha + p2.alpha) * 0.5f;
+        }
+        if (!done) {
+            if (p1.cost <= p2.cost && p1.cost < p0.cost) {
+                res_code = 4;
+                res = p1.alpha;
+            } else if (p2.cost <= p1.cost && p2.cost < p0.cost) {
+                res_code = 4;
+                res = p2.alpha;
+            } else {
+                res_code = 5;
+                res = 0.f;
+            }
+        }
+    }
+    LF(ls_result, 0) = res_code;
+    LF(ls_it, 0) = ls_it;
+    return res;
+}
+
+// solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
+__device__ inline void wave_solve(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
+                                  int lane) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    const bool ws = nc > 0 && LF(is_warmstart, 0);
+    for (int d = 0; d < D; d++) LF(qacc, d) = ws ? LF(qacc_ws, d) : LF(acc_smooth, d);
+    for (int d1 = lane; d1 < D; d1 += kWave) {
+        int e = m.dof_entity[d1];
+        float ma = 0.f;
+        for (int d2 = m.entity_dof_start[e]; d2 < m.entity_dof_end[e]; d2++) ma = ma + LF(mass_mat, d1 * D + d2) * LF(qacc, d2);
+        LF(Ma, d1) = ma;
+    }
+    for (int c = lane; c < nc; c += kWave) {
+        float ja = -R.aref(c);
+        for (int d = 0; d < D; d++) ja = ja + R.jac(c, d) * LF(qacc, d);
+        R.Jaref(c) = ja;
+        R.active(c) = 0;  // the first update compares against a cleared previous active set
+    }
+    wave_sync();
+    float cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
+    LF(cost, 0) = cost;
+    int improved = nc > 0;
+    wave_hessian_cholesky(m, loc, R, lane);
+    wave_gradient(m, loc);
+    for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+    int n_iter = 0;
+    if (nc > 0) {
+        const float tol = (m.meaninertia * (float)max(1, D)) * m.tolerance;
+        for (int it = 0; it < m.iterations; it++) {
+            n_iter++;
+            float alpha = wave_linesearch(m, loc, R, lane);
+            if (fabsf(alpha) < m.eps) {
+                improved = 0;
+            } else {
+                for (int d = 0; d < D; d++) {
+                    LF(qacc, d) = LF(qacc, d) + LF(search, d) * alpha;
+                    LF(Ma, d) = LF(Ma, d) + LF(mv, d) * alpha;
+                }
+                for (int c = lane; c < nc; c += kWave) R.Jaref(c) = R.Jaref(c) + R.jv(c) * alpha;
+                wave_sync();
+                float prev = LF(cost, 0);
+                cost = wave_update_constraint(m, loc, R, lane, prev);
+                LF(cost, 0) = cost;
+                if (wave_cholesky_incremental(m, loc, R, lane)) wave_hessian_cholesky(m, loc, R, lane);
+                wave_gradient(m, loc);
+                float gn = 0.f;
+                for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
+                gn = sqrtf(gn);
+                improved = gn > tol && (prev - cost) > tol;
+                if (improved)
+                    for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+            }
+            if (!improved) break;
+        }
+    } else {
+        improved = 0;
+    }
+    LF(improved, 0) = improved;
+    LF(n_iter, 0) = n_iter;
+    for (int d = 0; d < D; d++) {
+        float qa = LF(qacc, d);
+        LF(acc, d) = qa;
+        LF(qf_constraint, d) = LF(qfrc_constraint, d);
+        LF(force, d) = LF(qf_smooth, d) + LF(qfrc_constraint, d);
+        LF(qacc_ws, d) = qa;
+        if (isnan(qa)) LF(errbits, 0) = LF(errbits, 0) | RS_ERR_INVALID_FORCE_NAN;
+    }
+    LF(is_warmstart, 0) = 1;
+    const int cs = LF(n_con_eq, 0) + LF(n_con_fl, 0);
+    const int ncont = LF(n_contacts, 0);
+    for (int i = lane; i < ncont; i += kWave) {
+        f3 nrm = ld3(st.contact_normal, i, st.B, b);
+        float fr = st.contact_friction[(size_t)i * st.B + b];
+        f3 d1, d2, f = zero3();
+        orthogonals(nrm, d1, d2);
+        for (int k = 0; k < 4; k++) {
+            f3 dd = (float)(2 * (k % 2) - 1) * (k < 2 ? d1 : d2);
+            f = f + (dd * fr - nrm) * R.force(i * 4 + k + cs);
+        }
+        st3(st.contact_force, i, st.B, b, f);
+    }
+    wave_sync();
+}
+
+#undef LF
+
+}  // namespace rs

@@ -31,8 +31,9 @@ ERR_INVALID_ACC_NAN = 1 << 3
 
 COUNT_KINDS = [
     "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
-    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT",
+    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK",
 ]
+ENV_MAJOR_KINDS = {"WORK"}
 
 
 class RigidStepError(RuntimeError):
@@ -42,6 +43,11 @@ class RigidStepError(RuntimeError):
 class _ArrayInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("is_int", ctypes.c_int32), ("count", ctypes.c_int32),
                 ("width", ctypes.c_int32)]
+
+
+class FusedConfig(ctypes.Structure):
+    _fields_ = [("envs_per_block", ctypes.c_int32), ("use_lds", ctypes.c_int32),
+                ("lds_bytes_per_env", ctypes.c_int32)]
 
 
 _lib = None
@@ -59,6 +65,9 @@ _SIGNATURES = {
     "rs_model_destroy": (None, [ctypes.c_void_p]),
     "rs_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_int64)]),
+    "rs_state_field_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "rs_fused_config_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FusedConfig)]),
+    "rs_fused_config_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FusedConfig)]),
     "rs_state_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "rs_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_void_p]),
@@ -142,15 +151,21 @@ class DeviceModel:
             _lib.rs_model_destroy(self.handle)
             self.handle = None
 
-    def count(self, kind: str) -> int:
-        m = self.model
-        return {
-            "ONE": 1, "LINKS": m.n_links, "JOINTS": m.n_joints, "DOFS": m.n_dofs, "QS": m.n_qs, "GEOMS": m.n_geoms,
-            "ENTITIES": m.n_entities, "EQUALITIES": m.n_equalities, "DOFS2": m.n_dofs**2, "GEOMS2": m.n_geoms**2,
-            "PAIRS": m.n_pairs, "SUPPORT": len(m.arrays["support_v"]), "CONTACTS": m.max_contact_pairs,
-            "CONSTRAINTS": m.len_constraints, "CONSTRAINTS_DOFS": m.len_constraints * m.n_dofs,
-            "BROAD": m.max_collision_pairs_broad, "SORT": 2 * m.n_geoms,
-        }[kind]
+    def field_rows(self, index: int) -> int:
+        rows = ctypes.c_int64()
+        _check(_lib.rs_state_field_rows(self.handle, index, ctypes.byref(rows)))
+        return rows.value
+
+    @property
+    def fused_config(self) -> dict:
+        c = FusedConfig()
+        _check(_lib.rs_fused_config_get(self.handle, ctypes.byref(c)))
+        return {"envs_per_block": c.envs_per_block, "use_lds": bool(c.use_lds),
+                "lds_bytes_per_env": c.lds_bytes_per_env}
+
+    def set_fused_config(self, envs_per_block: int, use_lds: bool):
+        c = FusedConfig(envs_per_block, int(use_lds), 0)
+        _check(_lib.rs_fused_config_set(self.handle, ctypes.byref(c)))
 
 
 class StateSlab:
@@ -168,10 +183,11 @@ class StateSlab:
         device = device or torch.device("cuda", torch.cuda.current_device())
         self.buffer = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
         self.fields = {}
-        for (name, is_int, kind, width), off in zip(self.directory, offs):
-            rows = max(dmodel.count(kind) * width, 1)
-            raw = self.buffer[off : off + rows * n_envs * 4]
-            self.fields[name] = raw.view(torch.int32 if is_int else torch.float32).view(rows, n_envs)
+        for i, ((name, is_int, kind, width), off) in enumerate(zip(self.directory, offs)):
+            rows = dmodel.field_rows(i)
+            raw = self.buffer[off : off + rows * n_envs * 4].view(torch.int32 if is_int else torch.float32)
+            # [rows, n_envs] planes; the fused kernel's work field is env-major
+            self.fields[name] = raw.view(n_envs, rows) if kind in ENV_MAJOR_KINDS else raw.view(rows, n_envs)
         self.init()
 
     @property
@@ -223,3 +239,39 @@ class StepGraph:
         if getattr(self, "handle", None) and _lib is not None:
             _lib.rs_graph_destroy(self.handle)
             self.handle = None
+
+
+def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=None) -> dict:
+    """Pick the fused kernel's launch shape (environments per workgroup, LDS or work-field private regions) for this
+    model and batch by timing ``steps`` substeps of each candidate on a copy of the slab; the slab itself is left
+    unchanged. The winner is stored in the model and returned with the timings (ms per substep)."""
+    dm = slab.dmodel
+    if candidates is None:
+        candidates = [(e, lds) for lds in (True, False) for e in (1, 2, 4, 8)]
+    saved = dm.fused_config
+    scratch = StateSlab.__new__(StateSlab)
+    scratch.__dict__.update(slab.__dict__)
+    scratch.buffer = slab.buffer.clone()
+    s = stream or torch.cuda.current_stream()
+    timings = {}
+    for epb, lds in candidates:
+        try:
+            dm.set_fused_config(epb, lds)
+        except RigidStepError:
+            continue
+        scratch.buffer.copy_(slab.buffer)
+        scratch.step(2, RS_MODE_FUSED, s)  # warm-up
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        scratch.step(steps, RS_MODE_FUSED, s)
+        e1.record(s)
+        e1.synchronize()
+        timings[(epb, lds)] = e0.elapsed_time(e1) / steps
+    del scratch
+    if not timings:
+        dm.set_fused_config(saved["envs_per_block"], saved["use_lds"])
+        raise RigidStepError("autotune: no runnable fused configuration")
+    best = min(timings, key=timings.get)
+    dm.set_fused_config(*best)
+    return {"envs_per_block": best[0], "use_lds": best[1], "ms_per_substep": timings[best],
+            "timings": {f"epb{e}_{'lds' if l else 'hbm'}": t for (e, l), t in timings.items()}}

@@ -71,7 +71,9 @@ enum {
 enum {
     RS_N_ONE = 0, RS_N_LINKS, RS_N_JOINTS, RS_N_DOFS, RS_N_QS, RS_N_GEOMS, RS_N_ENTITIES, RS_N_EQUALITIES,
     RS_N_DOFS2, RS_N_GEOMS2, RS_N_PAIRS, RS_N_SUPPORT, RS_N_CONTACTS, RS_N_CONSTRAINTS, RS_N_CONSTRAINTS_DOFS,
-    RS_N_BROAD, RS_N_SORT, RS_N_KINDS
+    RS_N_BROAD, RS_N_SORT,
+    RS_N_WORK,  /* per-environment working set of the fused substep (size from rs_state_field_rows) */
+    RS_N_KINDS
 };
 
 #define RS_MAX_MODEL_ARRAYS 96
@@ -117,9 +119,13 @@ void rs_model_destroy(rs_model* model);
 
 /* ------------------------------------------------------------------------------------------------ state slab */
 
-/* Directory of per-environment state fields: each is a [rows][n_envs] plane of float32/int32. */
+/* Directory of per-environment state fields: each is a [rows][n_envs] plane of float32/int32, except fields of kind
+ * RS_N_WORK, which are env-major [n_envs][rows] scratch of the fused substep (contents unspecified between calls). */
 int rs_state_field_count(void);
 int rs_state_field_info(int index, rs_array_info* out);
+
+/* Rows per environment of state field `index` for this model (count of its kind x width, at least 1). */
+int rs_state_field_rows(const rs_model* model, int index, int64_t* rows);
 
 /* Byte offset of every state field for n_envs environments, and the slab size. offsets has rs_state_field_count()
  * entries. */
@@ -145,6 +151,18 @@ int rs_reset_caches(const rs_model* model, void* state, int n_envs, const int32_
 
 /* Bitwise OR of the per-environment error words into *out (synchronises the stream). */
 int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_t* out, void* stream);
+
+/* Launch configuration of the fused substep for one model: one 64-lane wavefront per environment,
+ * envs_per_block wavefronts per workgroup (1..8), and whether the environment's small fields are staged in LDS
+ * (use_lds = 1, only when they fit) or in the env-major work field. rs_fused_config_set rejects configurations the
+ * device cannot run. The model keeps the configuration for every later RS_MODE_FUSED step. */
+typedef struct rs_fused_config {
+    int32_t envs_per_block;
+    int32_t use_lds;
+    int32_t lds_bytes_per_env;  /* out: private-region size of one environment */
+} rs_fused_config;
+int rs_fused_config_get(const rs_model* model, rs_fused_config* out);
+int rs_fused_config_set(rs_model* model, const rs_fused_config* config);
 
 /* hipGraph capture of n_substeps fused substeps on a stream; replay with rs_graph_launch. */
 typedef struct rs_graph rs_graph;
