@@ -32,30 +32,26 @@ __device__ inline void clear_contacts(const DState& st, int b, int from, int to)
     }
 }
 
-// broadphase.py:141-396 _func_broad_phase_sap. (st, b) holds the AABBs, sort buffers, active list and pair list;
-// (gs, ge) holds the warm-start normal cache (the same view in the per-stage kernels).
-__device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, const DState& gs, int ge) {
-    const int G = m.G;
-    if (SF(first_time, 0)) {
-        int ib = 0;
-        for (int l = 0; l < m.L; l++)
-            for (int g = m.link_geom_start[l]; g < m.link_geom_end[l]; g++) {
-                SF(sort_value, 2 * ib) = SF(aabb_min, 3 * g);
-                SF(sort_i_g, 2 * ib) = g;
-                SF(sort_is_max, 2 * ib) = 0;
-                SF(sort_value, 2 * ib + 1) = SF(aabb_max, 3 * g);
-                SF(sort_i_g, 2 * ib + 1) = g;
-                SF(sort_is_max, 2 * ib + 1) = 1;
-                ib++;
-            }
-        SF(first_time, 0) = 0;
-    } else {
-        for (int i = 0; i < 2 * G; i++) {
-            int g = SF(sort_i_g, i);
-            SF(sort_value, i) = SF(sort_is_max, i) ? SF(aabb_max, 3 * g) : SF(aabb_min, 3 * g);
+// broadphase.py:141-200: first call fills the endpoint buffer in link/geom order; later calls refresh the values of
+// the previous order (warm start)
+__device__ inline void sap_init_endpoints(const DModel& m, const DState& st, int b) {
+    int ib = 0;
+    for (int l = 0; l < m.L; l++)
+        for (int g = m.link_geom_start[l]; g < m.link_geom_end[l]; g++) {
+            SF(sort_value, 2 * ib) = SF(aabb_min, 3 * g);
+            SF(sort_i_g, 2 * ib) = g;
+            SF(sort_is_max, 2 * ib) = 0;
+            SF(sort_value, 2 * ib + 1) = SF(aabb_max, 3 * g);
+            SF(sort_i_g, 2 * ib + 1) = g;
+            SF(sort_is_max, 2 * ib + 1) = 1;
+            ib++;
         }
-    }
-    for (int i = 1; i < 2 * G; i++) {
+    SF(first_time, 0) = 0;
+}
+
+// broadphase.py:200-230: insertion sort of the endpoints (stable: equal values keep their previous order)
+__device__ inline void sap_insertion_sort(const DState& st, int b, int n) {
+    for (int i = 1; i < n; i++) {
         float kv = SF(sort_value, i);
         int km = SF(sort_is_max, i), kg = SF(sort_i_g, i);
         int j = i - 1;
@@ -69,6 +65,21 @@ __device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, 
         SF(sort_is_max, j + 1) = km;
         SF(sort_i_g, j + 1) = kg;
     }
+}
+
+// broadphase.py:141-396 _func_broad_phase_sap. (st, b) holds the AABBs, sort buffers, active list and pair list;
+// (gs, ge) holds the warm-start normal cache (the same view in the per-stage kernels).
+__device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, const DState& gs, int ge) {
+    const int G = m.G;
+    if (SF(first_time, 0)) {
+        sap_init_endpoints(m, st, b);
+    } else {
+        for (int i = 0; i < 2 * G; i++) {
+            int g = SF(sort_i_g, i);
+            SF(sort_value, i) = SF(sort_is_max, i) ? SF(aabb_max, 3 * g) : SF(aabb_min, 3 * g);
+        }
+    }
+    sap_insertion_sort(st, b, 2 * G);
     int n_broad = 0, n_active = 0;
     for (int i = 0; i < 2 * G; i++) {
         if (!SF(sort_is_max, i)) {
@@ -140,7 +151,7 @@ __device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
         }
         // NaN directions map to cell (0, 1); keep the index inside the table in every case
         i = min(max(i, 0), res - 1);
-        const float* pp = &m.support_v[3 * (size_t)(start + i * res + j)];
+        cfptr pp = &m.support_v[3 * (size_t)(start + i * res + j)];
         f3 p = mk3(pp[0], pp[1], pp[2]);
         float dt = dot3(p, d);
         if (dt > dmax) {
@@ -154,7 +165,7 @@ __device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
 // mpr.py:147-202 support_driver
 __device__ inline f3 support_driver(const DModel& m, f3 d, int g, f3 pos, f4q quat) {
     int t = m.geom_type[g];
-    const float* data = &m.geom_data[7 * g];
+    cfptr data = &m.geom_data[7 * g];
     if (t == GT_SPHERE) return pos + d * data[0];
     if (t == GT_CAPSULE) {
         f3 axis = qrot(mk3(0.f, 0.f, 1.f), quat);
@@ -328,8 +339,8 @@ __device__ inline void guess_centers(const DModel& m, int ga, int gb, f3 pa, f4q
     ca = tr_point(mvec3(m.geom_center, ga), pa, qa);
     cb = tr_point(mvec3(m.geom_center, gb), pb, qb);
     if (fabsf(nws.x) > CCD_EPS || fabsf(nws.y) > CCD_EPS || fabsf(nws.z) > CCD_EPS) {
-        const float* A = &m.geom_init_aabb[24 * ga];
-        const float* Bx = &m.geom_init_aabb[24 * gb];
+        cfptr A = &m.geom_init_aabb[24 * ga];
+        cfptr Bx = &m.geom_init_aabb[24 * gb];
         f3 a0 = mk3(A[0], A[1], A[2]), a7 = mk3(A[21], A[22], A[23]);
         f3 b0 = mk3(Bx[0], Bx[1], Bx[2]), b7 = mk3(Bx[21], Bx[22], Bx[23]);
         ca = tr_point(0.5f * (a7 + a0), pa, qa);
@@ -378,10 +389,10 @@ __device__ inline void mpr_contact(const DModel& m, int ga, int gb, f3 nws, f3 p
 
 // ------------------------------------------------------------------------------------------------- contacts
 __device__ inline float geom_tolerance(const DModel& m, int ga, int gb, float tol) {
-    const float* B7 = &m.geom_init_aabb[24 * gb];
+    cfptr B7 = &m.geom_init_aabb[24 * gb];
     float s = norm3(mk3(B7[21] - B7[0], B7[22] - B7[1], B7[23] - B7[2]));
     if (m.geom_type[ga] != GT_PLANE) {
-        const float* A = &m.geom_init_aabb[24 * ga];
+        cfptr A = &m.geom_init_aabb[24 * ga];
         s = fminf(norm3(mk3(A[21] - A[0], A[22] - A[1], A[23] - A[2])), s);
     }
     return 0.5f * tol * s;
@@ -455,8 +466,8 @@ struct SlotSink {
 __device__ inline void contact_axes(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3& a0, f3& a1) {
     int g = gb;
     if (m.geom_type[ga] != GT_PLANE) {
-        const float* A = &m.geom_init_aabb[24 * ga + 21];
-        const float* Bx = &m.geom_init_aabb[24 * gb + 21];
+        cfptr A = &m.geom_init_aabb[24 * ga + 21];
+        cfptr Bx = &m.geom_init_aabb[24 * gb + 21];
         g = (A[0] * A[1] * A[2]) < (Bx[0] * Bx[1] * Bx[2]) ? ga : gb;
     }
     m33 R = quat_R(LD4(i_quat, m.geom_link[g]), m.eps);
@@ -678,7 +689,7 @@ __device__ inline void plane_box(const DModel& m, const DState& st, int b, int g
     if (!m.enable_multi_contact) return;
     int n_con = 1;
     float tol = geom_tolerance(m, ga, gb, m.mc_tolerance);
-    const float* d = &m.geom_data[7 * gb];
+    cfptr d = &m.geom_data[7 * gb];
     for (int c = 0; c < 8; c++) {
         if (n_con >= m.NCPP) break;
         f3 loc = mk3(((c & 4) ? 0.5f : -0.5f) * d[0], ((c & 2) ? 0.5f : -0.5f) * d[1], ((c & 1) ? 0.5f : -0.5f) * d[2]);

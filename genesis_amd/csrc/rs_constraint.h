@@ -28,8 +28,8 @@ __device__ inline int equality_rows(const DModel& m, const DState& st, int b, co
     int nc = 0;
     for (int e = 0; e < m.NE; e++) {
         if (m.eq_type[e] != EQ_JOINT) continue;
-        const float* sp = &m.eq_sol_params[7 * e];
-        const float* data = &m.eq_data[11 * e];
+        cfptr sp = &m.eq_sol_params[7 * e];
+        cfptr data = &m.eq_data[11 * e];
         int j1 = m.eq_obj1[e], j2 = m.eq_obj2[e];
         int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
         int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];

@@ -12,12 +12,18 @@
 namespace rs {
 
 // ------------------------------------------------------------------------------------------------- model view
+// Model arrays are read-only while kernels run, so they live in the constant address space: reads at addresses that
+// are the same on every lane become scalar loads.
+#define RS_CONST __attribute__((address_space(4)))
+typedef const float RS_CONST* cfptr;
+typedef const int32_t RS_CONST* ciptr;
+
 struct DModel {
     int L, J, D, Q, G, E, NE, P, NSUP, KMAX, CMAX, BROAD, NCPP;
     float dt, gx, gy, gz, tolerance, ls_tolerance, meaninertia, mc_perturbation, mc_tolerance, eps;
     int iterations, ls_iterations, enable_collision, enable_joint_limit, disable_constraint, enable_multi_contact;
     int box_box_detection, integrator, solver_type;
-#define RS_DM_PTR(name, is_int, kind, width) const typename std::conditional<is_int, int32_t, float>::type* name;
+#define RS_DM_PTR(name, is_int, kind, width) const typename std::conditional<is_int, int32_t, float>::type RS_CONST* name;
     RS_MODEL_ARRAYS(RS_DM_PTR)
 #undef RS_DM_PTR
 };
@@ -205,8 +211,8 @@ __device__ __forceinline__ void orthogonals(f3 a, f3& o1, f3& o2) {
     o1 = cross3(b, a);
     o2 = b;
 }
-__device__ __forceinline__ void imp_aref(const float* p, float neg_pen, float vel, float pos, float& imp,
-                                         float& aref) {
+template <class P>
+__device__ __forceinline__ void imp_aref(P p, float neg_pen, float vel, float pos, float& imp, float& aref) {
     float tc = p[0], dr = p[1], dmin = p[2], dmax = p[3], width = p[4], mid = p[5], power = p[6];
     float x = fabsf(neg_pen) / width;
     float ia = (1.f / powf(mid, power - 1.f)) * powf(x, power);
@@ -222,9 +228,9 @@ __device__ __forceinline__ void imp_aref(const float* p, float neg_pen, float ve
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
 // ------------------------------------------------------------------------------------- model accessors
-__device__ __forceinline__ f3 mvec3(const float* a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
-__device__ __forceinline__ f4q mquat(const float* a, int i) { return {a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]}; }
-__device__ __forceinline__ m33 mmat3(const float* a, int i) {
+__device__ __forceinline__ f3 mvec3(cfptr a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+__device__ __forceinline__ f4q mquat(cfptr a, int i) { return {a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]}; }
+__device__ __forceinline__ m33 mmat3(cfptr a, int i) {
     m33 r;
 #pragma unroll
     for (int k = 0; k < 9; k++) r.a[k / 3][k % 3] = a[9 * i + k];

@@ -324,7 +324,7 @@ __device__ inline void dev_torque_passive(const DModel& m, const DState& st, int
         for (int d = ds; d < de; d++) {
             float force = 0.f;
             int mode = SF(ctrl_mode, d);
-            const float* bb = &m.dof_act_bias[3 * d];
+            cfptr bb = &m.dof_act_bias[3 * d];
             if (mode == CTRL_FORCE) {
                 force = SF(ctrl_force, d);
             } else if (mode == CTRL_VELOCITY) {
@@ -349,7 +349,7 @@ __device__ inline void dev_torque_passive(const DModel& m, const DState& st, int
             }
             for (int k = 0; k < 3; k++) {
                 int d = ds + 3 + k;
-                const float* bb = &m.dof_act_bias[3 * d];
+                cfptr bb = &m.dof_act_bias[3 * d];
                 float g = m.dof_act_gain[d];
                 float force = g * rv[k] + bb[0] + (g + bb[1]) * SF(dof_pos, d) + bb[2] * (SF(vel, d) - SF(ctrl_vel, d));
                 SF(qf_applied, d) = clampf(force, m.dof_force_range[2 * d], m.dof_force_range[2 * d + 1]);
@@ -485,7 +485,7 @@ __device__ inline void dev_aabb(const DModel& m, const DState& st, int b, int g)
     f4q q = LD4(geoms_quat, g);
     f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);
     for (int c = 0; c < 8; c++) {
-        const float* a = &m.geom_init_aabb[24 * g + 3 * c];
+        cfptr a = &m.geom_init_aabb[24 * g + 3 * c];
         f3 cp = tr_point(mk3(a[0], a[1], a[2]), p, q);
         for (int k = 0; k < 3; k++) {
             lo[k] = fminf(lo[k], cp[k]);

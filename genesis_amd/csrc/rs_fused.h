@@ -11,6 +11,101 @@
 
 namespace rs {
 
+// broadphase.py:141-396 on one wavefront, producing the same pair list in the same order as broadphase_sap.
+// The endpoints are sorted by rank (rank = number of smaller values + number of equal values earlier in the previous
+// order, i.e. the stable sort that the insertion sort computes); NaN values fall back to the insertion sort itself.
+// In the sequential sweep, the active list when the min endpoint at sorted position p is reached holds exactly the
+// geoms whose min position is below p and whose max position is above p, in min-position order; so lane p walks those
+// candidates itself, counts its pairs, and a prefix sum over positions gives each lane its slice of the pair list.
+// The active_buffer field holds each geom's max position here (it is scratch in both paths).
+__device__ inline void wave_broadphase(const DModel& m, const DState& loc, const DState& st, int b, int lane) {
+    const int G = m.G, N = 2 * G;
+    if (loc.first_time[0]) {
+        sap_init_endpoints(m, loc, 0);
+    } else {
+        for (int i = lane; i < N; i += kWave) {
+            const int g = loc.sort_i_g[i];
+            loc.sort_value[i] = loc.sort_is_max[i] ? loc.aabb_max[3 * g] : loc.aabb_min[3 * g];
+        }
+    }
+    wave_sync();
+    bool nan = false;
+    for (int i = lane; i < N; i += kWave) nan |= isnan(loc.sort_value[i]);
+    if (wave_ballot(nan) != 0 || N > 2 * kWave) {
+        sap_insertion_sort(loc, 0, N);
+    } else {
+        float v[2];
+        int ig[2], mx[2], rank[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = lane + h * kWave;
+            rank[h] = -1;
+            if (i < N) {
+                v[h] = loc.sort_value[i];
+                ig[h] = loc.sort_i_g[i];
+                mx[h] = loc.sort_is_max[i];
+                int r = 0;
+                for (int j = 0; j < N; j++) {
+                    const float vj = loc.sort_value[j];
+                    r += (vj < v[h]) || (vj == v[h] && j < i);
+                }
+                rank[h] = r;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (rank[h] >= 0) {
+                loc.sort_value[rank[h]] = v[h];
+                loc.sort_i_g[rank[h]] = ig[h];
+                loc.sort_is_max[rank[h]] = mx[h];
+            }
+    }
+    wave_sync();
+    for (int p = lane; p < N; p += kWave)
+        if (loc.sort_is_max[p]) loc.active_buffer[loc.sort_i_g[p]] = p;
+    wave_sync();
+    int placed = 0;
+    for (int p0 = 0; p0 < N; p0 += kWave) {
+        const int p = p0 + lane;
+        const int gi = (p < N && !loc.sort_is_max[p]) ? loc.sort_i_g[p] : -1;
+        int cnt = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            int first = 0;
+            if (pass == 1) {
+                int total;
+                first = placed + wave_excl_scan(cnt, lane, total);
+                placed += total;
+            }
+            int k = 0;
+            for (int q = 0; gi >= 0 && q < p; q++) {
+                if (loc.sort_is_max[q]) continue;
+                const int g = loc.sort_i_g[q];
+                if (loc.active_buffer[g] < p) continue;  // its max endpoint came first: no longer active
+                const int ga = min(g, gi), gb = max(g, gi);
+                const int pidx = m.collision_pair_idx[ga * G + gb];
+                if (pidx == -1) continue;
+                if (!aabb_overlap(loc, 0, ga, gb)) {
+                    if (pass == 0) st3(st.cache_normal, pidx, st.B, b, zero3());
+                    continue;
+                }
+                if (pass == 0) {
+                    cnt++;
+                } else {
+                    const int idx = first + k++;
+                    if (idx < m.BROAD) {
+                        loc.broad_pairs[2 * idx] = ga;
+                        loc.broad_pairs[2 * idx + 1] = gb;
+                    }
+                }
+            }
+        }
+    }
+    if (placed > m.BROAD) loc.errbits[0] = loc.errbits[0] | RS_ERR_OVERFLOW_CANDIDATE_CONTACTS;
+    loc.n_broad[0] = min(placed, m.BROAD);
+    wave_sync();
+}
+
 // narrowphase.py:1988-2189 on one wavefront. Lanes take broadphase pairs; a pair's contacts go to its lane's slot
 // array (capacity n_contacts_per_pair) and are then compacted into the contact rows with a prefix sum, in pair order,
 // convex pairs first and plane-box pairs second - the order in which the sequential narrowphase appends them.
@@ -60,7 +155,7 @@ __device__ inline void wave_narrowphase(const DModel& m, const DState& loc, cons
 template <bool kLds>
 __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedArgs fa) {
     extern __shared__ float lds_env[];
-    const int w = (int)(threadIdx.x >> 6), lane = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = lane_id();
     // consecutive environments on one XCD (workgroups are dealt round-robin over the 8 XCDs), so that the planes'
     // cache lines, shared by neighbouring environments, stay in one L2
     const int nblk = (int)gridDim.x;
@@ -73,7 +168,7 @@ __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedAr
     const DState loc = make_local<kLds>(st, base, fa);
     const WorkRows R{wk + fa.off_jac, wk + fa.off_rowf, wk + fa.off_tmp, m.CMAX, m.D};
 
-    stage_local<true>(m, st, loc, b, lane);
+    stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
     wave_sync();
     // kernel_step_1: CRBA, L^T D L, actuation + passive + bias forces, acc_smooth (uniform on every lane)
     dev_mass_matrix(m, loc, 0);
@@ -88,8 +183,7 @@ __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedAr
         if (m.P > 0) {
             const int n_old = loc.n_contacts[0];
             for (int i = lane; i < n_old; i += kWave) clear_contacts(st, b, i, i + 1);
-            broadphase_sap(m, loc, 0, st, b);
-            wave_sync();
+            wave_broadphase(m, loc, st, b, lane);
             wave_narrowphase(m, loc, st, b, wk + fa.off_pc, lane);
         }
     }
@@ -103,7 +197,7 @@ __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedAr
     dev_integrate(m, loc, 0);
     dev_kinematics(m, loc, 0, false);
     wave_sync();
-    stage_local<false>(m, st, loc, b, lane);
+    stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
 }
 
 }  // namespace rs

@@ -67,6 +67,7 @@ struct rs_graph {
 };
 
 static int64_t work_floats(const rs_model_header& h);
+static int64_t field_rows(const rs_model_header& h, int i);
 
 static int64_t count_of(const rs_model_header& h, int kind) {
     switch (kind) {
@@ -92,18 +93,26 @@ static int64_t count_of(const rs_model_header& h, int kind) {
     }
 }
 
+// rows per environment of state field i (at least 1, so that every plane is addressable even for empty models)
+static int64_t field_rows(const rs_model_header& h, int i) {
+    int64_t rows = count_of(h, k_state_fields[i].count) * k_state_fields[i].width;
+    return rows < 1 ? 1 : rows;
+}
+
 // Private-region offsets of the local fields and the env-major work layout of the fused kernel (floats, each
 // sub-array 16-byte aligned, each environment's slice 256-byte aligned).
 static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     auto up4 = [](int64_t x) { return (x + 3) / 4 * 4; };
+    // local fields come first in the directory; the private region mirrors their packed planes row for row
     int64_t off = 0;
     for (int i = 0; i < k_n_state_fields && i < kMaxStateFields; i++) {
         fa->loc_off[i] = -1;
         if (!is_local_kind(k_state_fields[i].count)) continue;
         fa->loc_off[i] = (int)off;
-        int64_t rows = count_of(h, k_state_fields[i].count) * k_state_fields[i].width;
-        off += up4(rows < 1 ? 1 : rows);
+        off += field_rows(h, i);
     }
+    fa->n_local = (int)off;
+    off = up4(off);
     fa->loc_floats = (int)off;
     const int64_t C = h.len_constraints > 0 ? h.len_constraints : 1, D = h.n_dofs;
     fa->off_loc = 0;
@@ -122,15 +131,17 @@ static int64_t work_floats(const rs_model_header& h) {
 }
 
 // ------------------------------------------------------------------------------------------------ state views
+// Local-kind fields (first in the directory) are packed back to back; every other plane starts 256-byte aligned.
 static void layout(const rs_model_header& h, int n_envs, int64_t* offs, int64_t* total) {
     int64_t off = 0;
     for (int i = 0; i < k_n_state_fields; i++) {
-        int64_t rows = count_of(h, k_state_fields[i].count) * k_state_fields[i].width;
-        if (rows < 1) rows = 1;  // keep every plane addressable even for empty models
+        const int64_t bytes = field_rows(h, i) * n_envs * 4;
+        const bool local = is_local_kind(k_state_fields[i].count);
+        if (!local) off = (off + 255) / 256 * 256;
         offs[i] = off;
-        off += (rows * n_envs * 4 + 255) / 256 * 256;
+        off += bytes;
     }
-    *total = off;
+    *total = (off + 255) / 256 * 256;
 }
 
 static DState make_state(const rs_model* m, void* base, int n_envs) {
@@ -395,8 +406,7 @@ int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
 
 int rs_state_field_rows(const rs_model* m, int i, int64_t* rows) {
     if (!m || !rows || i < 0 || i >= k_n_state_fields) return fail(RS_E_INVALID, "rs_state_field_rows: bad argument");
-    int64_t r = count_of(m->h, k_state_fields[i].count) * k_state_fields[i].width;
-    *rows = r < 1 ? 1 : r;
+    *rows = field_rows(m->h, i);
     return RS_OK;
 }
 

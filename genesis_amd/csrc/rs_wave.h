@@ -23,33 +23,11 @@ __host__ __device__ constexpr bool is_local_kind(int kind) {
            kind == RS_N_BROAD;
 }
 
-__host__ __device__ inline int64_t kind_count(const DModel& m, int kind) {
-    switch (kind) {
-        case RS_N_ONE: return 1;
-        case RS_N_LINKS: return m.L;
-        case RS_N_JOINTS: return m.J;
-        case RS_N_DOFS: return m.D;
-        case RS_N_QS: return m.Q;
-        case RS_N_GEOMS: return m.G;
-        case RS_N_ENTITIES: return m.E;
-        case RS_N_EQUALITIES: return m.NE;
-        case RS_N_DOFS2: return (int64_t)m.D * m.D;
-        case RS_N_GEOMS2: return (int64_t)m.G * m.G;
-        case RS_N_PAIRS: return m.P;
-        case RS_N_SUPPORT: return m.NSUP;
-        case RS_N_CONTACTS: return m.KMAX;
-        case RS_N_CONSTRAINTS: return m.CMAX;
-        case RS_N_CONSTRAINTS_DOFS: return (int64_t)m.CMAX * m.D;
-        case RS_N_BROAD: return m.BROAD;
-        case RS_N_SORT: return 2 * (int64_t)m.G;
-        default: return 0;
-    }
-}
-
 // Launch parameters of the fused kernel; offsets are in floats.
 struct FusedArgs {
     int epb;           // environments (wavefronts) per workgroup
-    int loc_floats;    // private region per environment
+    int n_local;       // local rows staged per environment (the packed local planes)
+    int loc_floats;    // private region per environment (n_local rounded up to 16 bytes)
     int64_t work_stride;  // floats of the work field per environment
     int off_loc, off_jac, off_rowf, off_tmp, off_pc;
     int loc_off[kMaxStateFields];  // private-region offset of each state field (local kinds only)
@@ -58,15 +36,6 @@ struct FusedArgs {
 // per-row arrays of the constraint store (each n_constraints floats, after the jacobian)
 enum { RW_DIAG = 0, RW_AREF, RW_EFC_D, RW_FL, RW_JAREF, RW_JV, RW_FORCE, RW_ACTIVE, RW_PREV, RW_COUNT };
 constexpr int kTmpArrays = 9;
-
-struct Rows {
-    float* jac;  // [c * D + d]
-    float* f;    // [k * cap + c], k < RW_COUNT
-    float* tmp;  // [k * cap + c], k < kTmpArrays
-    int cap;
-    __device__ __forceinline__ float& at(int k, int c) const { return f[(size_t)k * cap + c]; }
-    __device__ __forceinline__ float& t(int k, int c) const { return tmp[(size_t)k * cap + c]; }
-};
 
 // ------------------------------------------------------------------------------------------- wave primitives
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
@@ -92,6 +61,22 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
     return x - v;
 }
 
+// acc + f(i0) + f(i0 + 1) + ... + f(i1 - 1), added left to right; up to U terms are loaded before they are added, so
+// that the loads of a batch overlap instead of each add waiting for its own load
+template <int U = 8, class F>
+__device__ __forceinline__ float seq_sum(float acc, int i0, int i1, F f) {
+    int i = i0;
+    for (; i + U <= i1; i += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = f(i + u);
+#pragma unroll
+        for (int u = 0; u < U; u++) acc = acc + v[u];
+    }
+    for (; i < i1; i++) acc = acc + f(i);
+    return acc;
+}
+
 // ---------------------------------------------------------------------------------------- private region
 template <bool kLds>
 __device__ __forceinline__ DState make_local(const DState& st, float* base, const FusedArgs& fa) {
@@ -106,20 +91,27 @@ __device__ __forceinline__ DState make_local(const DState& st, float* base, cons
     return l;
 }
 
-// copy the local fields of environment b between the planes and the private region
-template <bool kIn>
-__device__ __forceinline__ void stage_local(const DModel& m, const DState& st, const DState& loc, int b, int lane) {
+// Copy environment b's local rows between the packed local planes (which start at the slab base, the first local
+// field) and its private region: local row k is slab element k * n_envs + b. Rows are copied as raw 32-bit words.
+template <bool kIn, int kUnroll = 8>
+__device__ __forceinline__ void stage_local(const DState& st, uint32_t* priv, int n_local, int b, int lane) {
+    uint32_t* slab = (uint32_t*)(void*)st.qpos;
     const size_t B = (size_t)st.B;
-#define RS_X(name, is_int, kind, width)                                                          \
-    if constexpr (is_local_kind(kind)) {                                                         \
-        const int rows = (int)(kind_count(m, kind) * (width));                                   \
-        for (int r = lane; r < rows; r += kWave) {                                               \
-            if (kIn) loc.name[r] = st.name[(size_t)r * B + b];                                   \
-            else st.name[(size_t)r * B + b] = loc.name[r];                                       \
-        }                                                                                        \
+    int k = lane;
+    for (; k + (kUnroll - 1) * kWave < n_local; k += kUnroll * kWave) {
+        uint32_t v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) v[u] = kIn ? slab[(size_t)(k + u * kWave) * B + b] : priv[k + u * kWave];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            if (kIn) priv[k + u * kWave] = v[u];
+            else slab[(size_t)(k + u * kWave) * B + b] = v[u];
+        }
     }
-    RS_STATE_FIELDS(RS_X)
-#undef RS_X
+    for (; k < n_local; k += kWave) {
+        if (kIn) priv[k] = slab[(size_t)k * B + b];
+        else slab[(size_t)k * B + b] = priv[k];
+    }
 }
 
 }  // namespace rs

@@ -79,20 +79,17 @@ __device__ inline float wave_update_constraint(const DModel& m, const DState& lo
         R.tmp(1, c) = 0.5f * (ja * ja * Dc * (float)act);
     }
     wave_sync();
-    for (int d = lane; d < D; d += kWave) {
-        float q = 0.f;
-        for (int c = 0; c < nc; c++) q = q + R.jac(c, d) * R.force(c);
-        LF(qfrc_constraint, d) = q;
-    }
+    for (int d = lane; d < D; d += kWave)
+        LF(qfrc_constraint, d) = seq_sum(0.f, 0, nc, [&](int c) { return R.jac(c, d) * R.force(c); });
     // uniform: the cost in the reference's order (friction-loss terms, Gauss terms, quadratic row terms)
-    float cost = 0.f, gauss = 0.f;
-    for (int c = ne; c < nef; c++) cost = cost + R.tmp(0, c);
+    float cost = seq_sum(0.f, ne, nef, [&](int c) { return R.tmp(0, c); });
+    float gauss = 0.f;
     for (int d = 0; d < D; d++) {
         float v = 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
         gauss = gauss + v;
         cost = cost + v;
     }
-    for (int c = 0; c < nc; c++) cost = cost + R.tmp(1, c);
+    cost = seq_sum(cost, 0, nc, [&](int c) { return R.tmp(1, c); });
     LF(gauss, 0) = gauss;
     wave_sync();
     return cost;
@@ -110,7 +107,22 @@ __device__ inline void wave_hessian_cholesky(const DModel& m, const DState& loc,
         while (i * (i + 1) / 2 > e) i--;
         const int j = e - i * (i + 1) / 2;
         float h = 0.f;
-        for (int c = 0; c < nc; c++) {
+        constexpr int U = 8;
+        int c = 0;
+        for (; c + U <= nc; c += U) {
+            float v[U];
+            bool use[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                float ji = R.jac(c + u, i);
+                use[u] = fabsf(ji) > eps;
+                v[u] = R.jac(c + u, j) * ji * R.efc_D(c + u) * (float)R.active(c + u);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (use[u]) h = h + v[u];
+        }
+        for (; c < nc; c++) {
             float ji = R.jac(c, i);
             if (fabsf(ji) > eps) h = h + R.jac(c, j) * ji * R.efc_D(c) * (float)R.active(c);
         }
@@ -199,8 +211,7 @@ __device__ inline LsP wave_ls_init(const DModel& m, const DState& loc, const Wor
         LF(mv, i) = mv;
     }
     for (int c = lane; c < nc; c += kWave) {
-        float jv = 0.f;
-        for (int d = 0; d < D; d++) jv = jv + R.jac(c, d) * LF(search, d);
+        const float jv = seq_sum(0.f, 0, D, [&](int d) { return R.jac(c, d) * LF(search, d); });
         R.jv(c) = jv;
         float ja = R.Jaref(c), Dc = R.efc_D(c);
         float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
@@ -235,13 +246,14 @@ __device__ inline LsP wave_ls_init(const DModel& m, const DState& loc, const Wor
     LF(quad_gauss, 1) = g1;
     LF(quad_gauss, 2) = g2;
     float t0 = gauss, t1 = g1, t2 = g2, e0 = 0.f, e1 = 0.f, e2 = 0.f;
-    for (int c = 0; c < nc; c++) {
+    for (int c = 0; c < ne; c++) {
         float f0 = R.tmp(0, c), f1 = R.tmp(1, c), f2 = R.tmp(2, c);
-        if (c < ne) {
-            e0 = e0 + f0; e1 = e1 + f1; e2 = e2 + f2;
-        }
+        e0 = e0 + f0; e1 = e1 + f1; e2 = e2 + f2;
         t0 = t0 + f0; t1 = t1 + f1; t2 = t2 + f2;
     }
+    t0 = seq_sum(t0, ne, nc, [&](int c) { return R.tmp(0, c); });
+    t1 = seq_sum(t1, ne, nc, [&](int c) { return R.tmp(1, c); });
+    t2 = seq_sum(t2, ne, nc, [&](int c) { return R.tmp(2, c); });
     LF(eq_sum, 0) = e0;
     LF(eq_sum, 1) = e1;
     LF(eq_sum, 2) = e2;
@@ -287,12 +299,9 @@ __device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const Wo
     const float b0 = LF(quad_gauss, 0) + LF(eq_sum, 0), b1 = LF(quad_gauss, 1) + LF(eq_sum, 1),
                 b2 = LF(quad_gauss, 2) + LF(eq_sum, 2);
     for (int k = 0; k < n; k++) {
-        float t0 = b0, t1 = b1, t2 = b2;
-        for (int c = ne; c < nc; c++) {
-            t0 = t0 + R.tmp(3 * k, c);
-            t1 = t1 + R.tmp(3 * k + 1, c);
-            t2 = t2 + R.tmp(3 * k + 2, c);
-        }
+        const float t0 = seq_sum(b0, ne, nc, [&](int c) { return R.tmp(3 * k, c); });
+        const float t1 = seq_sum(b1, ne, nc, [&](int c) { return R.tmp(3 * k + 1, c); });
+        const float t2 = seq_sum(b2, ne, nc, [&](int c) { return R.tmp(3 * k + 2, c); });
         float a = al[k];
         float h = 2.f * t2;
         if (h <= 0.f) h = m.eps;
@@ -416,9 +425,7 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
         LF(Ma, d1) = ma;
     }
     for (int c = lane; c < nc; c += kWave) {
-        float ja = -R.aref(c);
-        for (int d = 0; d < D; d++) ja = ja + R.jac(c, d) * LF(qacc, d);
-        R.Jaref(c) = ja;
+        R.Jaref(c) = seq_sum(-R.aref(c), 0, D, [&](int d) { return R.jac(c, d) * LF(qacc, d); });
         R.active(c) = 0;  // the first update compares against a cleared previous active set
     }
     wave_sync();

@@ -183,21 +183,22 @@ class MJCFLoader:
         quat = self._orientation(b, "body", cls)
         inertial = b.find("inertial")
         if inertial is None:
-            raise NotImplementedError(f"body '{b.attrib.get('name')}' has no <inertial>")
-        mass = float(inertial.attrib["mass"])
-        ipos = _floats(inertial.attrib.get("pos", "0 0 0"))
-        iquat = self._orientation(inertial, "inertial", cls)
-        if "fullinertia" in inertial.attrib:
-            ixx, iyy, izz, ixy, ixz, iyz = _floats(inertial.attrib["fullinertia"])
-            I = np.array([[ixx, ixy, ixz], [ixy, iyy, iyz], [ixz, iyz, izz]])
-            I = tf.quat_to_R(iquat) @ I @ tf.quat_to_R(iquat).T
-            evals, evecs = np.linalg.eigh(I)
-            if np.linalg.det(evecs) < 0:
-                evecs[:, 2] = -evecs[:, 2]
-            iquat = tf.R_to_quat(evecs)
-            inertia = np.diag(evals)
+            mass, ipos, iquat, inertia = self._inertia_from_geoms(b, cls)
         else:
-            inertia = np.diag(_floats(inertial.attrib["diaginertia"]))
+            mass = float(inertial.attrib["mass"])
+            ipos = _floats(inertial.attrib.get("pos", "0 0 0"))
+            iquat = self._orientation(inertial, "inertial", cls)
+            if "fullinertia" in inertial.attrib:
+                ixx, iyy, izz, ixy, ixz, iyz = _floats(inertial.attrib["fullinertia"])
+                I = np.array([[ixx, ixy, ixz], [ixy, iyy, iyz], [ixz, iyz, izz]])
+                I = tf.quat_to_R(iquat) @ I @ tf.quat_to_R(iquat).T
+                evals, evecs = np.linalg.eigh(I)
+                if np.linalg.det(evecs) < 0:
+                    evecs[:, 2] = -evecs[:, 2]
+                iquat = tf.R_to_quat(evecs)
+                inertia = np.diag(evals)
+            else:
+                inertia = np.diag(_floats(inertial.attrib["diaginertia"]))
         link = LinkSpec(
             name=b.attrib.get("name", ""),
             parent=parent,
@@ -218,6 +219,54 @@ class MJCFLoader:
             if spec is not None:
                 link.geoms.append(spec)
         return link
+
+    def _inertia_from_geoms(self, b, cls):
+        """MuJoCo's inertiafromgeom: every geom of the body contributes density x volume (or its mass attribute)
+        and its inertia about the body frame; the sum is diagonalised into a principal inertial frame."""
+        total, first, second = 0.0, np.zeros(3), np.zeros((3, 3))
+        parts = []
+        for g in b.findall("geom"):
+            tname = self._attr(g, "geom", cls, "type", "sphere")
+            size = _floats(self._attr(g, "geom", cls, "size", "0 0 0"), 3)
+            pos = _floats(self._attr(g, "geom", cls, "pos", "0 0 0"))
+            quat = self._orientation(g, "geom", cls)
+            if tname == "box":
+                a, bb, c = size
+                vol = 8.0 * a * bb * c
+                unit = np.diag([bb * bb + c * c, a * a + c * c, a * a + bb * bb]) / 3.0
+            elif tname == "sphere":
+                r = size[0]
+                vol = 4.0 / 3.0 * np.pi * r**3
+                unit = np.eye(3) * 0.4 * r * r
+            elif tname == "capsule":
+                r, L = size[0], 2.0 * size[1]
+                v_cyl, v_sph = np.pi * r * r * L, 4.0 / 3.0 * np.pi * r**3
+                vol = v_cyl + v_sph
+                ixx = (v_cyl * (3 * r * r + L * L) / 12 + v_sph * (0.4 * r * r + 0.25 * L * L + 3 / 8 * r * L)) / vol
+                unit = np.diag([ixx, ixx, (0.5 * v_cyl * r * r + 0.4 * v_sph * r * r) / vol])
+            elif tname == "mesh":
+                com, mquat, verts, faces = self._mesh(self._attr(g, "geom", cls, "mesh"))
+                vol, _, inertia = mu.mass_properties(verts, faces)
+                pos, quat = tf.compose(pos, quat, com, mquat)
+                unit = inertia / max(vol, 1e-18)
+            else:
+                continue
+            m_attr = self._attr(g, "geom", cls, "mass")
+            mass = float(m_attr) if m_attr is not None else float(self._attr(g, "geom", cls, "density", "1000")) * vol
+            R = tf.quat_to_R(quat)
+            parts.append((mass, pos, R @ (mass * unit) @ R.T))
+            total += mass
+            first += mass * pos
+        if total <= 0.0:
+            return 0.0, np.zeros(3), np.array([1.0, 0.0, 0.0, 0.0]), np.zeros((3, 3))
+        com = first / total
+        for mass, pos, I in parts:
+            d = pos - com
+            second += I + mass * (np.dot(d, d) * np.eye(3) - np.outer(d, d))
+        evals, evecs = np.linalg.eigh(second)
+        if np.linalg.det(evecs) < 0:
+            evecs[:, 2] = -evecs[:, 2]
+        return total, com, tf.R_to_quat(evecs), np.diag(evals)
 
     def _joint(self, j, cls):
         jtype = _JOINT_TYPES[self._attr(j, "joint", cls, "type", "hinge")]

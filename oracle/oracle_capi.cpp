@@ -186,6 +186,20 @@ void* orc_create(const rs_model_header* hdr, const void* blob, int precision, in
     return h;
 }
 
+// opts = {dt, gx, gy, gz, tolerance, ls_tolerance, meaninertia, mc_perturbation, mc_tolerance}
+int orc_set_options(void* h, const double* o) {
+    return DISPATCH(h, {
+        c.m.dt = o[0];
+        for (int k = 0; k < 3; k++) c.m.gravity[k] = o[1 + k];
+        c.m.tolerance = o[4];
+        c.m.ls_tolerance = o[5];
+        c.m.meaninertia = o[6];
+        c.m.mc_perturbation = o[7];
+        c.m.mc_tolerance = o[8];
+        return 0;
+    });
+}
+
 void orc_destroy(void* h) {
     Handle* hh = reinterpret_cast<Handle*>(h);
     if (hh->precision == 64) delete reinterpret_cast<Context<double>*>(hh->ctx);

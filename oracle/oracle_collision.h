@@ -554,7 +554,7 @@ template <class T> void convex_convex_contact(const Model<T>& m, Env<T>& s, int 
     int ta = m.geom_type[ga], tb = m.geom_type[gb];
     if (ta == GT_PLANE && tb == GT_BOX) return;
     const T EPS = m.EPS;
-    const T mc_pert = T(m.h.mc_perturbation), mc_tol = T(m.h.mc_tolerance);
+    const T mc_pert = T(m.mc_perturbation), mc_tol = T(m.mc_tolerance);
     bool multi = m.h.enable_multi_contact && ta != GT_SPHERE && ta != GT_ELLIPSOID && tb != GT_SPHERE &&
                  tb != GT_ELLIPSOID;
     T tolerance = compute_tolerance(m, ga, gb, mc_tol);
@@ -659,7 +659,7 @@ template <class T> void plane_box_contact(const Model<T>& m, Env<T>& s, int ga, 
     if (!m.h.enable_multi_contact) return;
     int n_con = 1;
     V3<T> cp0 = cp;
-    T tol = compute_tolerance(m, ga, gb, T(m.h.mc_tolerance));
+    T tol = compute_tolerance(m, ga, gb, T(m.mc_tolerance));
     const T* d = &m.geom_data[7 * gb];
     for (int c = 0; c < 8; c++) {
         if (n_con >= m.h.n_contacts_per_pair) break;

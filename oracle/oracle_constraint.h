@@ -383,8 +383,8 @@ template <class T> T linesearch(const Model<T>& m, Env<T>& s) {
     T snorm = 0;
     for (int d = 0; d < D; d++) snorm = snorm + s.search[d] * s.search[d];
     snorm = std::sqrt(snorm);
-    T scale = T(m.h.meaninertia) * T(std::max(1, D));
-    T gtol = T(m.h.tolerance) * T(m.h.ls_tolerance) * snorm * scale;
+    T scale = T(m.meaninertia) * T(std::max(1, D));
+    T gtol = T(m.tolerance) * T(m.ls_tolerance) * snorm * scale;
     s.gtol = gtol;
     s.ls_it = 0;
     s.ls_result = 0;
@@ -495,7 +495,7 @@ template <class T> void constraint_solve(const Model<T>& m, Env<T>& s) {
                 update_constraint(m, s, s.qacc, s.Ma, s.cost);
                 if (hessian_cholesky_incremental(m, s)) hessian_cholesky_direct(m, s);
                 update_gradient(m, s);
-                T tol = (T(m.h.meaninertia) * T(std::max(1, D))) * T(m.h.tolerance);
+                T tol = (T(m.meaninertia) * T(std::max(1, D))) * T(m.tolerance);
                 T improvement = s.prev_cost - s.cost;
                 T gn = 0;
                 for (int d = 0; d < D; d++) gn = gn + s.grad[d] * s.grad[d];

@@ -251,7 +251,7 @@ template <class T> void compute_mass_matrix(const Model<T>& m, Env<T>& s, bool i
     }
     for (int d = 0; d < D; d++) s.mass_mat[d * D + d] = s.mass_mat[d * D + d] + m.dof_armature[d];
     if (implicit_damping) {
-        T dt = T(m.h.dt);
+        T dt = T(m.dt);
         for (int d = 0; d < D; d++) {
             s.mass_mat[d * D + d] = s.mass_mat[d * D + d] + m.dof_damping[d] * dt;
             if (s.ctrl_mode[d] <= CTRL_VELOCITY) s.mass_mat[d * D + d] = s.mass_mat[d * D + d] - m.dof_act_bias[3 * d + 2] * dt;
@@ -365,7 +365,7 @@ template <class T> void torque_and_passive(const Model<T>& m, Env<T>& s) {
 // forward_dynamics.py:1186-1285 func_update_acc
 template <class T> void update_acc(const Model<T>& m, Env<T>& s, bool update_cacc) {
     using E = Env<T>;
-    V3<T> g = v3(T(m.h.gravity[0]), T(m.h.gravity[1]), T(m.h.gravity[2]));
+    V3<T> g = v3(T(m.gravity[0]), T(m.gravity[1]), T(m.gravity[2]));
     for (int i_e = 0; i_e < m.E; i_e++) {
         T gc = T(1) - m.entity_gravity_compensation[i_e];
         for (int i_l = m.entity_link_start[i_e]; i_l < m.entity_link_end[i_e]; i_l++) {
@@ -465,7 +465,7 @@ template <class T> void update_aabbs(const Model<T>& m, Env<T>& s) {
 
 // forward_dynamics.py:1567-1707 func_integrate + abd/diff.py:26-55 func_copy_next_to_curr
 template <class T> void integrate(const Model<T>& m, Env<T>& s) {
-    T dt = T(m.h.dt);
+    T dt = T(m.dt);
     std::vector<T> vel_next(m.D), qpos_next(m.Q);
     for (int d = 0; d < m.D; d++) vel_next[d] = s.vel[d] + s.acc[d] * dt;
     for (int i_l = 0; i_l < m.L; i_l++) {

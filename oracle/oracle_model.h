@@ -21,6 +21,9 @@ constexpr int EQ_CONNECT = 0, EQ_WELD = 1, EQ_JOINT = 2;
 template <class T> struct Model {
     rs_model_header h;
     T EPS;
+    // real-valued options in double (the header carries them as float32 for the device; the FP64 restatement takes
+    // the exact values through orc_set_options)
+    double dt, gravity[3], tolerance, ls_tolerance, meaninertia, mc_perturbation, mc_tolerance;
     int L, J, D, Q, G, E, NE, P, NSUP;
     int KMAX, CMAX, BROAD;
 #define ORC_DECL(name, is_int, kind, width) std::vector<typename std::conditional<is_int, int32_t, T>::type> name;
@@ -63,6 +66,13 @@ template <class T> bool load_model(Model<T>& m, const rs_model_header* h, const 
     m.L = h->n_links; m.J = h->n_joints; m.D = h->n_dofs; m.Q = h->n_qs; m.G = h->n_geoms;
     m.E = h->n_entities; m.NE = h->n_equalities; m.P = h->n_pairs; m.NSUP = h->n_support;
     m.KMAX = h->max_contact_pairs; m.CMAX = h->len_constraints; m.BROAD = h->max_collision_pairs_broad;
+    m.dt = h->dt;
+    for (int k = 0; k < 3; k++) m.gravity[k] = h->gravity[k];
+    m.tolerance = h->tolerance;
+    m.ls_tolerance = h->ls_tolerance;
+    m.meaninertia = h->meaninertia;
+    m.mc_perturbation = h->mc_perturbation;
+    m.mc_tolerance = h->mc_tolerance;
     int idx = 0;
 #define ORC_LOAD(name, is_int, kind, width)                                                   \
     {                                                                                         \

@@ -35,6 +35,8 @@ def load():
         lib.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         lib.orc_forward_kinematics.argtypes = [ctypes.c_void_p]
+        lib.orc_set_options.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_set_options.restype = ctypes.c_int
         lib.orc_n_envs.argtypes = [ctypes.c_void_p]
         _lib = lib
     return _lib
@@ -53,6 +55,11 @@ class Oracle:
         self.n_envs = n_envs
         self.precision = precision
         self.model = model
+        # exact (double) option values; the header holds float32 copies for the device
+        opt = model.options
+        vals = np.array([opt.dt, *opt.gravity, opt.tolerance, opt.ls_tolerance, model.meaninertia, 1e-2, 1e-2],
+                        np.float64)
+        lib.orc_set_options(self.h, vals.ctypes.data)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
