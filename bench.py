@@ -1,8 +1,10 @@
-"""Benchmark: env-steps/s of the batched rigid step on Franka x 4096 (one MI355X), fused hipGraph path vs plain.
+"""Benchmark: env-steps/s of the batched rigid step on one MI355X, fused hipGraph path vs plain per-stage kernels.
 
-Workload (one "step" = one scene.step() of all environments): plane + Franka Panda (9 DoF, convex-hull collision,
-self-collision on), dt = 0.01 s, FP32, 4096 environments, initial joint positions uniform in the joint limits
-(PCG64 seed 0), PD position targets resampled uniformly in the limits every 100 steps (PCG64 seed 1000 + k).
+Default workload (the headline metric): Franka Panda x 4096 - plane + bundled Panda MJCF (9 DoF, convex-hull
+collision geoms, self-collision on), dt = 0.01 s, FP32, initial joint positions uniform in the joint limits (PCG64
+seed 0), PD position targets resampled uniformly in the limits every 100 steps (PCG64 seed 1000 + k).
+Other workloads (``--workload``): ``boxpile`` (20 free boxes x 1024 envs, dt 0.01) and ``quadruped`` (18-DoF
+quadruped x 8192 envs, dt 0.005, PD targets around the standing pose).
 All inputs are resident in HBM before the timed region; targets are swapped in with device copies.
 
 Prints one JSON line (rank 0). ``--gpus N`` with N > 1 is not supported: this build runs on one GPU.
@@ -24,61 +26,93 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 from genesis_amd import runtime as rt  # noqa: E402
-from genesis_amd.scenes import franka_initial_qpos, franka_model, franka_targets  # noqa: E402
+from genesis_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+BLOCK = 100  # substeps per captured graph (and per target block)
+
+
+class Workload:
+    def __init__(self, name):
+        self.name = name
+        if name == "franka":
+            self.model = scenes.franka_model()
+            self.B, self.steps = 4096, 1000
+            self.desc = "Franka Panda x{B}, plane, dt=0.01, PD targets resampled every 100 steps"
+        elif name == "boxpile":
+            self.model = scenes.box_pile_model()
+            self.B, self.steps = 1024, 500
+            self.desc = "box pile (20 free boxes) x{B}, plane, dt=0.01"
+        elif name == "quadruped":
+            self.model = scenes.quadruped_model()
+            self.B, self.steps = 8192, 1000
+            self.desc = "quadruped (18 DoF) x{B}, plane, dt=0.005, PD targets around standing pose"
+        else:
+            raise SystemExit(f"unknown workload {name}")
+
+    def init_state(self, slab, B):
+        m = self.model
+        if self.name == "franka":
+            slab["qpos"].copy_(torch.as_tensor(scenes.franka_initial_qpos(m, B).T, dtype=torch.float32))
+            slab["ctrl_mode"].fill_(0)
+        elif self.name == "quadruped":
+            slab["qpos"].copy_(torch.as_tensor(scenes.quadruped_initial_qpos(m, B).T, dtype=torch.float32))
+            slab["ctrl_mode"].copy_(torch.as_tensor(scenes.quadruped_ctrl_mode(m, B).T, dtype=torch.int32))
+        slab.forward_kinematics()
+
+    def targets(self, B, k):
+        m = self.model
+        if self.name == "franka":
+            return scenes.franka_targets(m, B, k)
+        if self.name == "quadruped":
+            return scenes.quadruped_targets(m, B, k)
+        return None
 
 
 def algorithmic_bytes_per_env_step(model, mean_overlap_pairs):
-    """Persistent per-env state read+written once per step (FP32/int32), Franka in POSITION mode:
-    qpos r+w 8Q, vel r+w 8D, acc w 4D, qacc_ws r+w 8D, ctrl_mode/ctrl_pos/ctrl_vel r 12D, errno r+w 8,
-    n_contacts w 4, is_warmstart r+w 2, plus the warm-start normal cache r+w (24 B) of each AABB-overlapping pair."""
+    """Persistent per-env state read+written once per step (FP32/int32): qpos r+w 8Q, vel r+w 8D, acc w 4D,
+    qacc_ws r+w 8D, ctrl_mode/ctrl_pos/ctrl_vel r 12D, errno r+w 8, n_contacts w 4, is_warmstart r+w 2, plus the
+    warm-start normal cache r+w (24 B) of each AABB-overlapping pair."""
     Q, D = model.n_qs, model.n_dofs
     return 8 * Q + 8 * D + 4 * D + 8 * D + 12 * D + 8 + 4 + 2 + 24 * mean_overlap_pairs
 
 
-def make_slab(model, dm, B):
+def make_slab(wl, dm, B):
     slab = rt.StateSlab(dm, B)
-    slab["qpos"].copy_(torch.as_tensor(franka_initial_qpos(model, B).T, dtype=torch.float32))
-    slab["ctrl_mode"].fill_(0)
-    slab.forward_kinematics()
+    wl.init_state(slab, B)
     return slab
 
 
-def run_path(model, dm, B, steps, warmup, mode, targets, use_graph, block=100):
-    """Returns (elapsed_s, per-step ms list, mean n_broad, slab)."""
-    slab = make_slab(model, dm, B)
+def run_path(wl, dm, B, steps, warmup, mode, targets, use_graph, n_lat=200):
+    """Returns (wall_s, gpu_ms, per-substep latencies ms, mean n_broad, slab)."""
+    slab = make_slab(wl, dm, B)
     stream = torch.cuda.Stream()
-    n_blocks = len(targets)
     with torch.cuda.stream(stream):
-        graph = rt.StepGraph(slab, block, mode, stream) if use_graph else None
+        graph = rt.StepGraph(slab, BLOCK, mode, stream) if use_graph else None
 
         def do_block(k, n):
-            slab["ctrl_pos"].copy_(targets[k % n_blocks])
-            if graph is not None and n == block:
+            if targets is not None:
+                slab["ctrl_pos"].copy_(targets[k % len(targets)])
+            if graph is not None and n == BLOCK:
                 graph.launch()
             else:
                 slab.step(n, mode, stream)
 
-        # warmup
-        w = warmup
-        k = 0
+        w, k = warmup, 0
         while w > 0:
-            n = min(block, w)
+            n = min(BLOCK, w)
             do_block(k, n)
             w -= n
             k += 1
         stream.synchronize()
-        # timed region
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        s = steps
-        k = 0
+        s, k = steps, 0
         while s > 0:
-            n = min(block, s)
+            n = min(BLOCK, s)
             do_block(k, n)
             s -= n
             k += 1
@@ -87,9 +121,9 @@ def run_path(model, dm, B, steps, warmup, mode, targets, use_graph, block=100):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         gpu_ms = ev0.elapsed_time(ev1)
-        # per-step latency: one substep at a time, each bracketed by events on the stream
+        # per-substep latency: one substep at a time, each bracketed by events on the launch stream
         lat = []
-        for i in range(min(200, steps)):
+        for _ in range(min(n_lat, steps)):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(stream)
@@ -118,103 +152,124 @@ def hbm_copy_peak_gbps():
     return 2 * n * 4 / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(model, n_envs=128, steps=200):
-    """CPU restatement (FP32, same step) on a bounded sample of the workload: n_envs x steps, 1 thread."""
+def cpu_baseline(wl, n_envs=128, steps=200):
+    """CPU restatement (FP32, same step) on a bounded sample of the workload: n_envs x steps, at 1 thread and at
+    min(nproc, 16) threads."""
     from tests.oracle_lib import Oracle
 
-    o = Oracle(model, n_envs, precision=32)
-    o.set("qpos", franka_initial_qpos(model, n_envs))
-    o.set("ctrl_mode", np.zeros((n_envs, model.n_dofs)))
-    o.set("ctrl_pos", franka_targets(model, n_envs, 0))
-    o.forward_kinematics()
-    t0 = time.perf_counter()
-    o.step(steps, 1)
-    dt1 = time.perf_counter() - t0
+    m = wl.model
+    out = []
     nthreads = min(os.cpu_count() or 1, 16)
-    o2 = Oracle(model, n_envs, precision=32)
-    o2.set("qpos", franka_initial_qpos(model, n_envs))
-    o2.set("ctrl_mode", np.zeros((n_envs, model.n_dofs)))
-    o2.set("ctrl_pos", franka_targets(model, n_envs, 0))
-    o2.forward_kinematics()
-    t0 = time.perf_counter()
-    o2.step(steps, nthreads)
-    dtn = time.perf_counter() - t0
-    return n_envs * steps / dt1, n_envs * steps / dtn, nthreads
+    for threads in (1, nthreads):
+        o = Oracle(m, n_envs, precision=32)
+        if wl.name == "franka":
+            o.set("qpos", scenes.franka_initial_qpos(m, n_envs))
+            o.set("ctrl_mode", np.zeros((n_envs, m.n_dofs)))
+        elif wl.name == "quadruped":
+            o.set("qpos", scenes.quadruped_initial_qpos(m, n_envs))
+            o.set("ctrl_mode", scenes.quadruped_ctrl_mode(m, n_envs))
+        t = wl.targets(n_envs, 0)
+        if t is not None:
+            o.set("ctrl_pos", t)
+        o.forward_kinematics()
+        t0 = time.perf_counter()
+        o.step(steps, threads)
+        out.append(n_envs * steps / (time.perf_counter() - t0))
+    return out[0], out[1], nthreads
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--envs", type=int, default=4096)
+    ap.add_argument("--envs", type=int, default=None)
+    ap.add_argument("--workload", default="franka", choices=["franka", "boxpile", "quadruped"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU "
-              "(environments are independent; see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(environments are independent; DESIGN.md section 7 describes how the path would shard)",
+              file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     torch.cuda.set_device(0)
-    B = args.envs
-    model = franka_model()
+    wl = Workload(args.workload)
+    B = args.envs or wl.B
+    steps = args.steps or wl.steps
+    model = wl.model
     dm = rt.DeviceModel(model)
-    n_blocks = max(1, (args.steps + args.warmup + 99) // 100)
-    targets = [torch.as_tensor(franka_targets(model, B, k).T, dtype=torch.float32, device="cuda").contiguous()
-               for k in range(n_blocks)]
-    print(f"[bench] model D={model.n_dofs} G={model.n_geoms} pairs={model.n_pairs}; B={B}", file=sys.stderr,
-          flush=True)
-    wall_f, gpu_ms_f, lat_f, nb_f, slab = run_path(model, dm, B, args.steps, args.warmup, rt.RS_MODE_FUSED,
-                                                   targets, use_graph=True)
+    n_blocks = max(1, (steps + args.warmup + BLOCK - 1) // BLOCK)
+    targets = None
+    if wl.targets(1, 0) is not None:
+        targets = [torch.as_tensor(wl.targets(B, k).T, dtype=torch.float32, device="cuda").contiguous()
+                   for k in range(n_blocks)]
+    print(f"[bench] {args.workload}: D={model.n_dofs} G={model.n_geoms} pairs={model.n_pairs}; B={B} steps={steps}",
+          file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        probe = make_slab(wl, dm, B)
+        probe.step(20, rt.RS_MODE_FUSED)  # tune on a state with contacts, not the initial one
+        tune = rt.autotune_fused(probe, steps=10)
+        del probe
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+    wall_f, gpu_ms_f, lat_f, nb_f, slab = run_path(wl, dm, B, steps, args.warmup, rt.RS_MODE_FUSED, targets,
+                                                   use_graph=True)
     errno = slab.errno()
-    print(f"[bench] fused: {wall_f:.3f}s for {args.steps} steps", file=sys.stderr, flush=True)
+    print(f"[bench] fused: {wall_f:.3f}s for {steps} steps", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain:
-        wall_p, gpu_ms_p, lat_p, nb_p, _ = run_path(model, dm, B, args.steps, args.warmup, rt.RS_MODE_PLAIN,
-                                                    targets, use_graph=False)
-        plain = {"value": B * args.steps / wall_p, "ms_per_step": 1e3 * wall_p / args.steps,
+        wall_p, gpu_ms_p, lat_p, nb_p, _ = run_path(wl, dm, B, steps, args.warmup, rt.RS_MODE_PLAIN, targets,
+                                                    use_graph=False, n_lat=50)
+        plain = {"value": B * steps / wall_p, "ms_per_step": 1e3 * wall_p / steps,
                  "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))}
         print(f"[bench] plain: {wall_p:.3f}s", file=sys.stderr, flush=True)
-    value = B * args.steps / wall_f
+    value = B * steps / wall_f
     bytes_per_env_step = algorithmic_bytes_per_env_step(model, nb_f)
-    kernel_ms = gpu_ms_f / args.steps  # one fused launch per step
+    kernel_ms = gpu_ms_f / steps  # one fused launch per substep, events on the launch stream
     achieved = bytes_per_env_step * B / (kernel_ms * 1e-3) / 1e9
     out = {
         "metric": "env-steps/s at Franka×4096 (fused vs plain GPU, FP32); p50/p99 step latency",
         "value": value,
         "unit": "env-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall_f / args.steps,
+        "ms_per_step": 1e3 * wall_f / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"Franka Panda x{B}, plane, dt=0.01, PD targets resampled every 100 steps",
-                   "n_envs": B, "impl": "fused substep kernel, 100-step hipGraph"},
+        "config": {"workload": wl.desc.format(B=B), "n_envs": B,
+                   "impl": f"fused substep kernel (one wavefront per env), {BLOCK}-substep hipGraph",
+                   "fused_config": dm.fused_config},
         "p50_step_ms": float(np.percentile(lat_f, 50)),
         "p99_step_ms": float(np.percentile(lat_f, 99)),
         "plain": plain,
         "fused_over_plain": (value / plain["value"]) if plain else None,
+        "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "bytes_per_env_step": bytes_per_env_step, "mean_overlap_pairs": nb_f,
-                     "kernel": "k_fused_substep", "kernel_ms": kernel_ms},
+                     "kernel": "k_fused_wave", "kernel_ms": kernel_ms},
         "errno": errno,
     }
+    if args.workload != "franka":
+        out["metric"] = f"env-steps/s, {args.workload} (fused vs plain GPU, FP32); p50/p99 step latency"
     try:
         out["hbm_copy_GBps"] = hbm_copy_peak_gbps()
     except Exception as e:  # noqa: BLE001
         out["hbm_copy_GBps"] = None
         print(f"[bench] copy peak failed: {e}", file=sys.stderr)
     if not args.no_cpu_baseline and rank == 0:
-        v1, vn, nt = cpu_baseline(model)
+        v1, vn, nt = cpu_baseline(wl)
         out["cpu_baseline"] = {"value": v1, "unit": "env-steps/s", "cores": 1, "kind": "port",
-                               "sample": "Franka x128 envs x 200 steps, same seeds, FP32 C++ restatement",
-                               "value_all_cores": vn, "cores_all": nt, "cpu": platform.processor() or platform.machine()}
+                               "sample": f"{args.workload} x128 envs x 200 steps, same seeds, FP32 C++ restatement",
+                               "value_all_cores": vn, "cores_all": nt,
+                               "cpu": platform.processor() or platform.machine()}
     if rank == 0:
         print(json.dumps(out), flush=True)
 

@@ -94,7 +94,8 @@
     RS_SF(improved, 1, RS_N_ONE, 1)          \
     RS_SF(ls_it, 1, RS_N_ONE, 1)             \
     RS_SF(ls_result, 1, RS_N_ONE, 1)         \
-    RS_SF(n_iter, 1, RS_N_ONE, 1)
+    RS_SF(n_iter, 1, RS_N_ONE, 1)            \
+    RS_SF(wave_scratch, 0, RS_N_DOFS, 6)
 
 #define RS_GLOBAL_FIELDS(RS_SF)                      \
     RS_SF(cache_normal, 0, RS_N_PAIRS, 3)            \

@@ -7,6 +7,7 @@
 #include "rs_constraint.h"
 #include "rs_dyn.h"
 #include "rs_wave.h"
+#include "rs_wave_dyn.h"
 #include "rs_wave_solver.h"
 
 namespace rs {
@@ -152,17 +153,11 @@ __device__ inline void wave_narrowphase(const DModel& m, const DState& loc, cons
     wave_sync();
 }
 
+// One environment's substep on the calling wavefront (w = wavefront index in the workgroup).
 template <bool kLds>
-__global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedArgs fa) {
-    extern __shared__ float lds_env[];
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = lane_id();
-    // consecutive environments on one XCD (workgroups are dealt round-robin over the 8 XCDs), so that the planes'
-    // cache lines, shared by neighbouring environments, stay in one L2
-    const int nblk = (int)gridDim.x;
-    int blk = (int)blockIdx.x;
-    if ((nblk & 7) == 0) blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
-    const int b = blk * fa.epb + w;
-    if (b >= st.B) return;  // whole wavefronts only
+__device__ __forceinline__ void fused_env(const DModel& m, const DState& st, const FusedArgs& fa, int b, int w,
+                                          float* lds_env) {
+    const int lane = lane_id();
     float* wk = st.wk + (size_t)b * fa.work_stride;
     float* base = kLds ? lds_env + (size_t)w * fa.loc_floats : wk + fa.off_loc;
     const DState loc = make_local<kLds>(st, base, fa);
@@ -170,12 +165,12 @@ __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedAr
 
     stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
     wave_sync();
-    // kernel_step_1: CRBA, L^T D L, actuation + passive + bias forces, acc_smooth (uniform on every lane)
-    dev_mass_matrix(m, loc, 0);
-    dev_factor_mass(m, loc, 0);
-    dev_torque_passive(m, loc, 0);
-    dev_update_acc(m, loc, 0, false);
-    dev_rne_and_qacc(m, loc, 0);
+    // kernel_step_1: CRBA, L^T D L, actuation + passive + bias forces, acc_smooth
+    wave_mass_matrix(m, loc, lane);
+    wave_factor_mass(m, loc, lane);
+    wave_torque_passive(m, loc, lane);
+    wave_update_acc(m, loc, lane, false);
+    wave_rne_and_qacc(m, loc, lane);
     // Collider.detection
     if (m.enable_collision) {
         for (int g = lane; g < m.G; g += kWave) dev_aabb(m, loc, 0, g);
@@ -193,11 +188,49 @@ __global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedAr
         wave_solve(m, loc, st, b, R, lane);
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
-    dev_update_acc(m, loc, 0, true);
+    wave_update_acc(m, loc, lane, true);
     dev_integrate(m, loc, 0);
-    dev_kinematics(m, loc, 0, false);
     wave_sync();
+    wave_kinematics(m, loc, lane);
     stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
+}
+
+// Consecutive environments go to one XCD (workgroups are dealt round-robin over the 8 XCDs), so that the planes'
+// cache lines, shared by neighbouring environments, stay in one L2.
+__device__ __forceinline__ int xcd_block(int blk, int nblk) {
+    return (nblk & 7) == 0 ? (blk & 7) * (nblk >> 3) + (blk >> 3) : blk;
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedArgs fa) {
+    extern __shared__ float lds_env[];
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x) * fa.epb + w;
+    if (b >= st.B) return;  // whole wavefronts only
+    fused_env<kLds>(m, st, fa, b, w, lds_env);
+}
+
+// One batch of a shared launch: a model, its state slab, and the first workgroup that serves it.
+struct FusedSegment {
+    DModel m;
+    DState st;
+    FusedArgs fa;
+    int first_block, n_blocks;
+};
+
+// Several models and slabs in one launch (the serving loop's co-batched step). Every segment uses the launch's
+// environments-per-workgroup; a workgroup finds its segment by scanning the table (a few dozen entries at most).
+template <bool kLds>
+__global__ void __launch_bounds__(512) k_fused_multi(const FusedSegment* __restrict__ segs, int n_segs) {
+    extern __shared__ float lds_env[];
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int s = 0;
+    while (s + 1 < n_segs && segs[s + 1].first_block <= blk) s++;
+    const FusedSegment& g = segs[s];
+    const int b = (blk - g.first_block) * g.fa.epb + w;
+    if (b >= g.st.B) return;
+    fused_env<kLds>(g.m, g.st, g.fa, b, w, lds_env);
 }
 
 }  // namespace rs

@@ -61,6 +61,18 @@ struct rs_model {
     int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
 };
 
+// Shared launches of several (model, slab) batches: a ring of pinned host + device segment tables, each reused only
+// after the copy that last read it has completed.
+struct rs_batch {
+    static constexpr int kRing = 4;
+    int max_segments = 0;
+    int slot = 0;
+    FusedSegment* host[kRing] = {};
+    FusedSegment* dev[kRing] = {};
+    hipEvent_t done[kRing] = {};
+    bool pending[kRing] = {};
+};
+
 struct rs_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -512,6 +524,75 @@ int rs_graph_create(const rs_model* m, void* state, int n_envs, int n_substeps, 
 int rs_graph_launch(rs_graph* g, void* stream) {
     if (!g) return fail(RS_E_INVALID, "rs_graph_launch: null graph");
     HIP_OK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return RS_OK;
+}
+
+int rs_batch_create(int max_segments, rs_batch** out) {
+    if (max_segments < 1 || !out) return fail(RS_E_INVALID, "rs_batch_create: bad argument");
+    rs_batch* bt = new rs_batch();
+    bt->max_segments = max_segments;
+    for (int i = 0; i < rs_batch::kRing; i++) {
+        if (hipHostMalloc((void**)&bt->host[i], sizeof(FusedSegment) * max_segments, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void**)&bt->dev[i], sizeof(FusedSegment) * max_segments) != hipSuccess ||
+            hipEventCreateWithFlags(&bt->done[i], hipEventDisableTiming) != hipSuccess) {
+            rs_batch_destroy(bt);
+            return fail(RS_E_DEVICE, "rs_batch_create: allocation failed");
+        }
+    }
+    *out = bt;
+    return RS_OK;
+}
+
+void rs_batch_destroy(rs_batch* bt) {
+    if (!bt) return;
+    for (int i = 0; i < rs_batch::kRing; i++) {
+        if (bt->done[i]) {
+            (void)hipEventSynchronize(bt->done[i]);
+            (void)hipEventDestroy(bt->done[i]);
+        }
+        if (bt->dev[i]) (void)hipFree(bt->dev[i]);
+        if (bt->host[i]) (void)hipHostFree(bt->host[i]);
+    }
+    delete bt;
+}
+
+int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* const* states, const int* n_envs,
+                  void* stream) {
+    if (!bt || n < 1 || n > bt->max_segments || !models || !states || !n_envs)
+        return fail(RS_E_INVALID, "rs_batch_step: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int epb = 2;
+    int max_loc = 0;
+    for (int i = 0; i < n; i++) {
+        if (!models[i] || !states[i] || n_envs[i] < 1) return fail(RS_E_INVALID, "rs_batch_step: bad segment");
+        max_loc = max_loc > models[i]->fa.loc_floats ? max_loc : models[i]->fa.loc_floats;
+    }
+    int max_lds = models[0]->max_lds;
+    for (int i = 1; i < n; i++) max_lds = max_lds < models[i]->max_lds ? max_lds : models[i]->max_lds;
+    const bool use_lds = (int64_t)epb * max_loc * 4 <= max_lds;
+    const int k = bt->slot;
+    bt->slot = (bt->slot + 1) % rs_batch::kRing;
+    if (bt->pending[k]) HIP_OK(hipEventSynchronize(bt->done[k]));
+    FusedSegment* tab = bt->host[k];
+    int blocks = 0;
+    for (int i = 0; i < n; i++) {
+        tab[i].m = models[i]->dm;
+        tab[i].st = make_state(models[i], states[i], n_envs[i]);
+        tab[i].fa = models[i]->fa;
+        tab[i].fa.epb = epb;
+        tab[i].first_block = blocks;
+        tab[i].n_blocks = (n_envs[i] + epb - 1) / epb;
+        blocks += tab[i].n_blocks;
+    }
+    HIP_OK(hipMemcpyAsync(bt->dev[k], tab, sizeof(FusedSegment) * n, hipMemcpyHostToDevice, s));
+    HIP_OK(hipEventRecord(bt->done[k], s));
+    bt->pending[k] = true;
+    const dim3 grid((unsigned)blocks), block((unsigned)(kWave * epb));
+    if (use_lds)
+        hipLaunchKernelGGL(k_fused_multi<true>, grid, block, (size_t)epb * max_loc * 4, s, bt->dev[k], n);
+    else
+        hipLaunchKernelGGL(k_fused_multi<false>, grid, block, 0, s, bt->dev[k], n);
+    HIP_OK(hipGetLastError());
     return RS_OK;
 }
 

@@ -1,0 +1,384 @@
+// Smooth dynamics and kinematics of one environment on one wavefront (fused substep).
+// Same arithmetic, in the same order per value, as rs_dyn.h (abd/forward_kinematics.py, abd/forward_dynamics.py):
+// per-link, per-dof and per-entry work is split across lanes; accumulations along the kinematic tree (subtree
+// inertias, bias forces, spatial velocities and accelerations) are split by vector component, so that each lane walks
+// the tree in the reference's order for one component and no lane waits for another.
+#pragma once
+#include "rs_dyn.h"
+#include "rs_wave.h"
+
+namespace rs {
+
+#define LF(field, row) loc.field[(row)]
+
+// forward_dynamics.py:292-541 func_compute_mass_matrix
+__device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int lane) {
+    const int D = m.D, L = m.L;
+    for (int t = lane; t < L * 13; t += kWave) {
+        const int l = t / 13, k = t - 13 * (t / 13);
+        if (k < 9) LF(crb_inertial, 9 * l + k) = LF(cinr_inertial, 9 * l + k);
+        else if (k < 12) LF(crb_pos, 3 * l + k - 9) = LF(cinr_pos, 3 * l + k - 9);
+        else LF(crb_mass, l) = LF(cinr_mass, l);
+    }
+    wave_sync();
+    // subtree accumulation, leaves to root: lane k owns component k of every link
+    if (lane < 13) {
+        float* f = lane < 9 ? loc.crb_inertial : (lane < 12 ? loc.crb_pos : loc.crb_mass);
+        const int w = lane < 9 ? 9 : (lane < 12 ? 3 : 1), c = lane < 9 ? lane : (lane < 12 ? lane - 9 : 0);
+        for (int e = 0; e < m.E; e++) {
+            const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
+            for (int i = 0; i < l1 - l0; i++) {
+                const int l = l1 - 1 - i, p = m.link_parent[l];
+                if (p != -1) f[w * p + c] = f[w * p + c] + f[w * l + c];
+            }
+        }
+    }
+    wave_sync();
+    for (int d = lane; d < D; d += kWave) {
+        const int l = m.dof_link[d];
+        f3 fa, fv;
+        inertial_mul(ld3(loc.crb_pos, l, 1, 0), ld9(loc.crb_inertial, l, 1, 0), LF(crb_mass, l),
+                     ld3(loc.cdof_vel, d, 1, 0), ld3(loc.cdof_ang, d, 1, 0), fa, fv);
+        st3(loc.f_ang, d, 1, 0, fa);
+        st3(loc.f_vel, d, 1, 0, fv);
+    }
+    wave_sync();
+    for (int t = lane; t < D * D; t += kWave) {
+        const int i = t / D, j = t - D * (t / D);
+        const int e = m.dof_entity[i];
+        if (m.dof_entity[j] != e) continue;
+        const int a = i >= j ? i : j, c = i >= j ? j : i;  // the lower-triangle entry (a, c) carries the value
+        float v = (dot3(ld3(loc.f_ang, a, 1, 0), ld3(loc.cdof_ang, c, 1, 0)) +
+                   dot3(ld3(loc.f_vel, a, 1, 0), ld3(loc.cdof_vel, c, 1, 0))) *
+                  m.mass_parent_mask[a * D + c];
+        if (i == j) {
+            v = v + m.dof_armature[i];
+            if (m.integrator == 2) {
+                v = v + m.dof_damping[i] * m.dt;
+                if (LF(ctrl_mode, i) <= CTRL_VELOCITY) v = v - m.dof_act_bias[3 * i + 2] * m.dt;
+            }
+        }
+        LF(mass_mat, t) = v;
+    }
+    wave_sync();
+}
+
+// forward_dynamics.py:560-602 func_factor_mass: for each pivot i (last to first), the rows j < i are updated by
+// separate lanes; row i's entries are read before any lane rescales them
+__device__ inline void wave_factor_mass(const DModel& m, const DState& loc, int lane) {
+    const int D = m.D;
+    for (int e = 0; e < m.E; e++) {
+        const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e], n = d1 - d0;
+        for (int t = lane; t < n * n; t += kWave) {
+            const int i = d0 + t / n, j = d0 + (t - n * (t / n));
+            if (j <= i) LF(mass_L, i * D + j) = LF(mass_mat, i * D + j);
+        }
+        wave_sync();
+        if (n > kWave + 1) {  // more rows than lanes: the sequential elimination, uniformly
+            for (int ii = 0; ii < n; ii++) {
+                const int i = d1 - ii - 1;
+                const float dinv = 1.f / LF(mass_L, i * D + i);
+                LF(mass_D_inv, i) = dinv;
+                for (int jj = 0; jj < i - d0; jj++) {
+                    const int j = i - jj - 1;
+                    const float a = LF(mass_L, i * D + j) * dinv;
+                    for (int k = d0; k <= j; k++) LF(mass_L, j * D + k) = LF(mass_L, j * D + k) - a * LF(mass_L, i * D + k);
+                    LF(mass_L, i * D + j) = a;
+                }
+                LF(mass_L, i * D + i) = 1.f;
+            }
+            wave_sync();
+            continue;
+        }
+        for (int ii = 0; ii < n; ii++) {
+            const int i = d1 - ii - 1;
+            const float dinv = 1.f / LF(mass_L, i * D + i);
+            float a = 0.f;
+            const int j = d0 + lane;
+            if (j < i) {
+                a = LF(mass_L, i * D + j) * dinv;
+                for (int k = d0; k <= j; k++) LF(mass_L, j * D + k) = LF(mass_L, j * D + k) - a * LF(mass_L, i * D + k);
+            }
+            wave_sync();
+            if (j < i) LF(mass_L, i * D + j) = a;
+            LF(mass_D_inv, i) = dinv;
+            LF(mass_L, i * D + i) = 1.f;
+            wave_sync();
+        }
+    }
+}
+
+// forward_dynamics.py:956-1182 func_torque_and_passive_force, one lane per link (then per dof)
+__device__ inline void wave_torque_passive(const DModel& m, const DState& loc, int lane) {
+    for (int l = lane; l < m.L; l += kWave) {
+        const int ds = m.link_dof_start[l], de = m.link_dof_end[l];
+        if (de == ds) continue;
+        const int jt = m.joint_type[m.link_joint_start[l]];
+        for (int d = ds; d < de; d++) {
+            float force = 0.f;
+            const int mode = LF(ctrl_mode, d);
+            cfptr bb = &m.dof_act_bias[3 * d];
+            if (mode == CTRL_FORCE) {
+                force = LF(ctrl_force, d);
+            } else if (mode == CTRL_VELOCITY) {
+                force = -bb[2] * (LF(ctrl_vel, d) - LF(vel, d));
+            } else if (mode == CTRL_POSITION && !(jt == JT_FREE && d >= ds + 3)) {
+                float g = m.dof_act_gain[d], p = LF(dof_pos, d);
+                force = g * (LF(ctrl_pos, d) - p) + bb[0] + (g + bb[1]) * p + bb[2] * (LF(vel, d) - LF(ctrl_vel, d));
+            }
+            LF(qf_applied, d) = clampf(force, m.dof_force_range[2 * d], m.dof_force_range[2 * d + 1]);
+        }
+        if (jt == JT_FREE && (LF(ctrl_mode, ds + 3) == CTRL_POSITION || LF(ctrl_mode, ds + 4) == CTRL_POSITION ||
+                              LF(ctrl_mode, ds + 5) == CTRL_POSITION)) {
+            f3 xyz = mk3(LF(dof_pos, ds + 3), LF(dof_pos, ds + 4), LF(dof_pos, ds + 5));
+            f3 cxyz = mk3(LF(ctrl_pos, ds + 3), LF(ctrl_pos, ds + 4), LF(ctrl_pos, ds + 5));
+            f4q qd = qtrans(xyz_quat(cxyz), qinv(xyz_quat(xyz)));
+            f3 rv = mk3(qd.x, qd.y, qd.z);
+            float s2 = norm3(rv);
+            if (s2 > m.eps) {
+                float ang = 2.f * atan2f(s2, fabsf(qd.w));
+                float is = ang / s2;
+                rv = ((qd.w < 0.f ? -1.f : 1.f) * is) * rv;
+            }
+            for (int k = 0; k < 3; k++) {
+                int d = ds + 3 + k;
+                cfptr bb = &m.dof_act_bias[3 * d];
+                float g = m.dof_act_gain[d];
+                float force = g * rv[k] + bb[0] + (g + bb[1]) * LF(dof_pos, d) + bb[2] * (LF(vel, d) - LF(ctrl_vel, d));
+                LF(qf_applied, d) = clampf(force, m.dof_force_range[2 * d], m.dof_force_range[2 * d + 1]);
+            }
+        }
+    }
+    for (int d = lane; d < m.D; d += kWave) {
+        float p = -m.dof_damping[d] * LF(vel, d);
+        const int l = m.dof_link[d];
+        const int jt = m.joint_type[m.link_joint_start[l]];
+        if (jt != JT_FREE && jt != JT_FIXED) p = p + (-LF(dof_pos, d)) * m.dof_stiffness[d];
+        LF(qf_passive, d) = p;
+    }
+    wave_sync();
+}
+
+// forward_dynamics.py:1186-1285 func_update_acc: lane k < 6 owns component k of (cdd_vel, cdd_ang) and, with cacc,
+// lane 6 + k of (cacc_lin, cacc_ang), walking the links in order
+__device__ inline void wave_update_acc(const DModel& m, const DState& loc, int lane, bool cacc) {
+    const int nlanes = cacc ? 12 : 6;
+    if (lane < nlanes) {
+        const int c = lane % 3, grp = lane / 3;  // 0: cdd_vel, 1: cdd_ang, 2: cacc_lin, 3: cacc_ang
+        float* out = grp == 0 ? loc.cdd_vel : grp == 1 ? loc.cdd_ang : grp == 2 ? loc.cacc_lin : loc.cacc_ang;
+        const float* dd = (grp == 0 || grp == 2) ? loc.cdofd_vel : loc.cdofd_ang;
+        const float* dq = (grp == 0 || grp == 2) ? loc.cdof_vel : loc.cdof_ang;
+        const float g = c == 0 ? m.gx : (c == 1 ? m.gy : m.gz);
+        for (int e = 0; e < m.E; e++) {
+            const float gc = 1.f - m.entity_gravity_compensation[e];
+            for (int l = m.entity_link_start[e]; l < m.entity_link_end[e]; l++) {
+                const int p = m.link_parent[l];
+                float x = p == -1 ? (grp == 0 ? (-g) * gc : 0.f) : out[3 * p + c];
+                for (int d = m.link_dof_start[l]; d < m.link_dof_end[l]; d++) {
+                    const float lv = dd[3 * d + c] * LF(vel, d);
+                    if (grp < 2) x = x + lv;
+                    else x = x + (lv + dq[3 * d + c] * LF(acc, d));
+                }
+                out[3 * l + c] = x;
+            }
+        }
+    }
+    wave_sync();
+}
+
+// forward_dynamics.py:1289-1400 func_update_force, :1428-1486 func_bias_force, :1507-1563 func_compute_qacc
+__device__ inline void wave_rne_and_qacc(const DModel& m, const DState& loc, int lane) {
+    for (int l = lane; l < m.L; l += kWave) {
+        m33 I = ld9(loc.cinr_inertial, l, 1, 0);
+        f3 cp = ld3(loc.cinr_pos, l, 1, 0), cdv = ld3(loc.cd_vel, l, 1, 0), cda = ld3(loc.cd_ang, l, 1, 0);
+        float cm = LF(cinr_mass, l);
+        f3 f1a, f1v, f2a, f2v, f3a, f3v;
+        inertial_mul(cp, I, cm, ld3(loc.cdd_vel, l, 1, 0), ld3(loc.cdd_ang, l, 1, 0), f1a, f1v);
+        inertial_mul(cp, I, cm, cdv, cda, f2a, f2v);
+        motion_x_force(cda, cdv, f2a, f2v, f3a, f3v);
+        st3(loc.cfrc_vel, l, 1, 0, f1v + f3v + zero3() + zero3());
+        st3(loc.cfrc_ang, l, 1, 0, f1a + f3a + zero3() + zero3());
+    }
+    wave_sync();
+    if (lane < 6) {
+        float* f = lane < 3 ? loc.cfrc_vel : loc.cfrc_ang;
+        const int c = lane % 3;
+        for (int e = 0; e < m.E; e++) {
+            const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
+            for (int i = 0; i < l1 - l0; i++) {
+                const int l = l1 - 1 - i, p = m.link_parent[l];
+                if (p != -1) f[3 * p + c] = f[3 * p + c] + f[3 * l + c];
+            }
+        }
+    }
+    wave_sync();
+    for (int d = lane; d < m.D; d += kWave) {
+        const int l = m.dof_link[d];
+        f3 fa = ld3(loc.cfrc_ang, l, 1, 0), fv = ld3(loc.cfrc_vel, l, 1, 0);
+        float bias = dot3(ld3(loc.cdof_ang, d, 1, 0), fa) + dot3(ld3(loc.cdof_vel, d, 1, 0), fv);
+        LF(qf_bias, d) = bias;
+        float f = LF(qf_passive, d) - bias + LF(qf_applied, d);
+        LF(force, d) = f;
+        LF(qf_smooth, d) = f;
+    }
+    wave_sync();
+    dev_solve_mass(m, loc, 0, loc.force, loc.acc_smooth);
+    for (int d = 0; d < m.D; d++) LF(acc, d) = LF(acc_smooth, d);
+}
+
+// forward_kinematics.py:337-576 func_COM_links_entity
+__device__ inline void wave_com_links(const DModel& m, const DState& loc, int lane) {
+    for (int e = 0; e < m.E; e++) {
+        const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
+        for (int l = l0 + lane; l < l1; l += kWave) {
+            f3 ip;
+            f4q iq;
+            tr_pose(mvec3(m.link_inertial_pos, l) + zero3(), mquat(m.link_inertial_quat, l), ld3(loc.links_pos, l, 1, 0),
+                    ld4(loc.links_quat, l, 1, 0), ip, iq);
+            st3(loc.i_pos, l, 1, 0, ip);
+            st4(loc.i_quat, l, 1, 0, iq);
+            st3(loc.root_COM, l, 1, 0, zero3());
+            LF(crb_mass, l) = 0.f;
+        }
+        wave_sync();
+        // root sums in link order: lane 0 the mass, lanes 1..3 the weighted position
+        if (lane < 4) {
+            for (int l = l0; l < l1; l++) {
+                const float mass = m.link_inertial_mass[l] + 0.f;
+                const int r = m.link_root[l];
+                if (lane == 0) LF(crb_mass, r) = LF(crb_mass, r) + mass;
+                else LF(root_COM, 3 * r + lane - 1) = LF(root_COM, 3 * r + lane - 1) + mass * LF(i_pos, 3 * l + lane - 1);
+            }
+        }
+        wave_sync();
+        for (int l = l0 + lane; l < l1; l += kWave) {
+            const int r = m.link_root[l];
+            if (l == r) {
+                float ms = LF(crb_mass, l);
+                if (ms > m.eps) st3(loc.root_COM, l, 1, 0, ld3(loc.root_COM, l, 1, 0) / ms);
+                else st3(loc.root_COM, l, 1, 0, ld3(loc.i_pos, r, 1, 0));
+            }
+        }
+        wave_sync();
+        for (int l = l0 + lane; l < l1; l += kWave) {
+            const f3 rc = ld3(loc.root_COM, m.link_root[l], 1, 0);
+            f3 ipos = ld3(loc.i_pos, l, 1, 0) - rc;
+            st3(loc.i_pos, l, 1, 0, ipos);
+            float mass = m.link_inertial_mass[l] + 0.f;
+            m33 ci;
+            f3 cp;
+            inertia_offset(mmat3(m.link_inertial_i, l), mass, ipos, ld4(loc.i_quat, l, 1, 0), m.eps, ci, cp);
+            st9(loc.cinr_inertial, l, 1, 0, ci);
+            st3(loc.cinr_pos, l, 1, 0, cp);
+            LF(cinr_mass, l) = mass;
+        }
+        wave_sync();
+        for (int l = l0 + lane; l < l1; l += kWave) {
+            if (m.link_root[l] != l) st3(loc.root_COM, l, 1, 0, ld3(loc.root_COM, m.link_root[l], 1, 0));
+        }
+        for (int j = lane; j < m.J; j += kWave) {
+            const int l = m.joint_link[j];
+            if (l < l0 || l >= l1 || m.link_dof_end[l] == m.link_dof_start[l]) continue;
+            f3 off = ld3(loc.root_COM, m.link_root[l], 1, 0) - ld3(loc.xanchor, j, 1, 0);
+            const int jt = m.joint_type[j], ds = m.joint_dof_start[j];
+            f3 xa = ld3(loc.xaxis, j, 1, 0);
+            if (jt == JT_REVOLUTE) {
+                st3(loc.cdof_ang, ds, 1, 0, xa);
+                st3(loc.cdof_vel, ds, 1, 0, cross3(xa, off));
+            } else if (jt == JT_PRISMATIC) {
+                st3(loc.cdof_ang, ds, 1, 0, zero3());
+                st3(loc.cdof_vel, ds, 1, 0, xa);
+            } else if (jt == JT_SPHERICAL || jt == JT_FREE) {
+                int base = ds;
+                if (jt == JT_FREE) {
+                    for (int k = 0; k < 3; k++) {
+                        f3 e3 = zero3();
+                        e3[k] = 1.f;
+                        st3(loc.cdof_ang, ds + k, 1, 0, zero3());
+                        st3(loc.cdof_vel, ds + k, 1, 0, e3);
+                    }
+                    base = ds + 3;
+                }
+                m33 R = quat_R(ld4(loc.links_quat, l, 1, 0), m.eps);
+                for (int k = 0; k < 3; k++) {
+                    f3 row = mk3(R.a[0][k], R.a[1][k], R.a[2][k]);
+                    st3(loc.cdof_ang, base + k, 1, 0, row);
+                    st3(loc.cdof_vel, base + k, 1, 0, cross3(row, off));
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
+// forward_kinematics.py:993-1116 func_forward_velocity_entity. Lane k < 6 walks the links for component k of
+// (cd_vel, cd_ang) and records, for every dof, the (cd_ang, cd_vel) that its cdofd is taken against (the
+// velocity accumulated before the dof's joint - after the translational dofs for a free joint's rotations); then one
+// lane per dof forms cdofd.
+__device__ inline void wave_forward_velocity(const DModel& m, const DState& loc, int lane) {
+    float* pre = loc.wave_scratch;  // [d * 6 + (0..2: ang, 3..5: vel)]
+    if (lane < 6) {
+        const int c = lane % 3;
+        const bool ang = lane >= 3;
+        float* out = ang ? loc.cd_ang : loc.cd_vel;
+        const float* cd = ang ? loc.cdof_ang : loc.cdof_vel;
+        const int slot = ang ? c : 3 + c;
+        for (int l = 0; l < m.L; l++) {
+            const int p = m.link_parent[l];
+            float x = p != -1 ? out[3 * p + c] : 0.f;
+            for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
+                const int jt = m.joint_type[j], ds = m.joint_dof_start[j], de = m.joint_dof_end[j];
+                if (jt == JT_FREE) {
+                    for (int k = 0; k < 3; k++) x = x + cd[3 * (ds + k) + c] * LF(vel, ds + k);
+                    for (int k = 3; k < 6; k++) pre[6 * (ds + k) + slot] = x;
+                    for (int k = 3; k < 6; k++) x = x + cd[3 * (ds + k) + c] * LF(vel, ds + k);
+                } else {
+                    for (int d = ds; d < de; d++) pre[6 * d + slot] = x;
+                    for (int d = ds; d < de; d++) x = x + cd[3 * d + c] * LF(vel, d);
+                }
+            }
+            out[3 * l + c] = x;
+        }
+    }
+    wave_sync();
+    for (int d = lane; d < m.D; d += kWave) {
+        const int l = m.dof_link[d];
+        int j = m.link_joint_start[l];
+        while (j + 1 < m.link_joint_end[l] && m.joint_dof_end[j] <= d) j++;
+        f3 oa = zero3(), ov = zero3();
+        if (!(m.joint_type[j] == JT_FREE && d < m.joint_dof_start[j] + 3)) {
+            f3 ca = mk3(pre[6 * d], pre[6 * d + 1], pre[6 * d + 2]), cv = mk3(pre[6 * d + 3], pre[6 * d + 4], pre[6 * d + 5]);
+            motion_x_motion(ca, cv, ld3(loc.cdof_ang, d, 1, 0), ld3(loc.cdof_vel, d, 1, 0), oa, ov);
+        }
+        st3(loc.cdofd_ang, d, 1, 0, oa);
+        st3(loc.cdofd_vel, d, 1, 0, ov);
+    }
+    wave_sync();
+}
+
+// forward_kinematics.py:828-864 func_update_geoms_entity, one lane per geom
+__device__ inline void wave_update_geoms(const DModel& m, const DState& loc, int lane) {
+    for (int g = lane; g < m.G; g += kWave) {
+        if (m.geom_is_fixed[g]) continue;
+        const int l = m.geom_link[g];
+        f3 p;
+        f4q q;
+        tr_pose(mvec3(m.geom_pos, g), mquat(m.geom_quat, g), ld3(loc.links_pos, l, 1, 0), ld4(loc.links_quat, l, 1, 0), p,
+                q);
+        st3(loc.geoms_pos, g, 1, 0, p);
+        st4(loc.geoms_quat, g, 1, 0, q);
+    }
+}
+
+// forward kinematics (the chain walk runs on every lane), then COM, geoms and velocities split across lanes
+__device__ inline void wave_kinematics(const DModel& m, const DState& loc, int lane) {
+    dev_forward_kinematics(m, loc, 0);
+    wave_sync();
+    wave_com_links(m, loc, lane);
+    wave_update_geoms(m, loc, lane);
+    wave_forward_velocity(m, loc, lane);
+}
+
+#undef LF
+
+}  // namespace rs

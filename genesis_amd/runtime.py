@@ -81,6 +81,10 @@ _SIGNATURES = {
     "rs_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "rs_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "rs_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "rs_batch_destroy": (None, [ctypes.c_void_p]),
+    "rs_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
     "rs_graph_destroy": (None, [ctypes.c_void_p]),
 }
 
@@ -238,6 +242,29 @@ class StepGraph:
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
             _lib.rs_graph_destroy(self.handle)
+            self.handle = None
+
+
+class SharedLaunch:
+    """One fused substep of several slabs (of possibly different models) in a single kernel launch (rs_batch)."""
+
+    def __init__(self, max_segments: int = 64):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.rs_batch_create(max_segments, ctypes.byref(h)))
+        self.handle = h
+        self.max_segments = max_segments
+
+    def step(self, slabs, stream=None):
+        n = len(slabs)
+        models = (ctypes.c_void_p * n)(*[s.dmodel.handle.value for s in slabs])
+        states = (ctypes.c_void_p * n)(*[s.buffer.data_ptr() for s in slabs])
+        envs = (ctypes.c_int * n)(*[s.n_envs for s in slabs])
+        _check(_lib.rs_batch_step(self.handle, n, models, states, envs, _stream_handle(stream)))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.rs_batch_destroy(self.handle)
             self.handle = None
 
 

@@ -164,6 +164,17 @@ typedef struct rs_fused_config {
 int rs_fused_config_get(const rs_model* model, rs_fused_config* out);
 int rs_fused_config_set(rs_model* model, const rs_fused_config* config);
 
+/* Shared launches for serving: one fused substep of n independent batches - each a model, its state slab and its
+ * environment count, models may differ - in a single kernel launch. Each environment's result is the same as when
+ * its slab is stepped alone with rs_step(RS_MODE_FUSED). The batch object owns a ring of four small pinned/device
+ * segment tables (up to max_segments entries each); rs_batch_step waits only for the table copy it issued four calls
+ * earlier before reusing that table. No counterpart in the reference (its scenes step one at a time). */
+typedef struct rs_batch rs_batch;
+int rs_batch_create(int max_segments, rs_batch** out);
+void rs_batch_destroy(rs_batch* batch);
+int rs_batch_step(rs_batch* batch, int n, const rs_model* const* models, void* const* states, const int* n_envs,
+                  void* stream);
+
 /* hipGraph capture of n_substeps fused substeps on a stream; replay with rs_graph_launch. */
 typedef struct rs_graph rs_graph;
 int rs_graph_create(const rs_model* model, void* state, int n_envs, int n_substeps, int mode, void* stream,

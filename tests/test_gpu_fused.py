@@ -138,6 +138,37 @@ def test_fused_box_pile_equals_plain():
         assert d == 0.0, f"{name}: {d:.3e}"
 
 
+def test_fused_quadruped_equals_plain_and_tracks_oracle():
+    rt = _rt()
+    from genesis_amd.scenes import quadruped_ctrl_mode, quadruped_initial_qpos, quadruped_model, quadruped_targets
+
+    B = 128
+    model = quadruped_model()
+    dm = rt.DeviceModel(model)
+    a = rt.StateSlab(dm, B)
+    a["qpos"].copy_(torch.as_tensor(quadruped_initial_qpos(model, B).T, dtype=torch.float32))
+    a["ctrl_mode"].copy_(torch.as_tensor(quadruped_ctrl_mode(model, B).T, dtype=torch.int32))
+    a["ctrl_pos"].copy_(torch.as_tensor(quadruped_targets(model, B, 0).T, dtype=torch.float32))
+    a.forward_kinematics()
+    a.step(100, rt.RS_MODE_PLAIN)  # feet on the ground
+    torch.cuda.synchronize()
+    b = _twin(rt, dm, a)
+    oracle = Oracle(model, B, precision=32)
+    slab_to_oracle(a, oracle)
+    print(f"[fused] quadruped B={B} config={dm.fused_config}", flush=True)
+    a.step(1, rt.RS_MODE_PLAIN)
+    b.step(1, rt.RS_MODE_FUSED)
+    torch.cuda.synchronize()
+    oracle.step(1)
+    for name in OUTPUTS:
+        d = _max_abs(get_dev(a, name), get_dev(b, name))
+        assert d == 0.0, f"{name}: fused differs from per-stage by {d:.3e}"
+    assert np.array_equal(get_dev(b, "n_contacts"), oracle.get("n_contacts"))
+    for name in ("qpos", "vel"):
+        err = scaled_err(get_dev(b, name), oracle.get(name))
+        assert err < 5e-5, f"{name}: {err:.3e}"
+
+
 def test_autotune_keeps_state_and_picks_a_shape():
     rt = _rt()
     model, dm, slab = _franka(512, 10)
