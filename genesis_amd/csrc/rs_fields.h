@@ -3,11 +3,11 @@
 // consecutive addresses. Names follow the reference state structs (genesis/utils/array_class.py: DofsState,
 // LinksState, JointsState, GeomsState, ColliderState.contact_data, ConstraintState, RigidGlobalInfo.qpos/mass_mat).
 //
-// The small per-environment fields (RS_LOCAL_FIELDS: counts of links, joints, dofs, qs, geoms, entities, dofs^2,
-// SAP endpoints and broadphase pairs) come first and their planes are packed back to back, so that one environment's
+// The small per-environment fields (RS_LOCAL_FIELDS: counts of links, joints, dofs, qs, geoms, entities, dofs^2 and
+// SAP endpoints) come first and their planes are packed back to back, so that one environment's
 // local fields form a single strided sequence: local row k of environment b is element k * n_envs + b of the slab.
-// The fused kernel stages exactly that sequence into its private region. Fields sized by pairs, contacts and
-// constraint rows follow (RS_GLOBAL_FIELDS); the last one, wk, is the env-major working set of the fused kernel.
+// The fused kernel stages exactly that sequence into its private region. Fields sized by broadphase pairs, pairs,
+// contacts and constraint rows follow (RS_GLOBAL_FIELDS); the last one, wk, is the env-major working set of the fused kernel.
 #pragma once
 
 #define RS_LOCAL_FIELDS(RS_SF)               \
@@ -70,7 +70,6 @@
     RS_SF(vel_next, 0, RS_N_DOFS, 1)         \
     RS_SF(qpos_next, 0, RS_N_QS, 1)          \
     RS_SF(n_broad, 1, RS_N_ONE, 1)           \
-    RS_SF(broad_pairs, 1, RS_N_BROAD, 2)     \
     RS_SF(active_buffer, 1, RS_N_GEOMS, 1)   \
     RS_SF(n_contacts, 1, RS_N_ONE, 1)        \
     RS_SF(n_con, 1, RS_N_ONE, 1)             \
@@ -98,6 +97,7 @@
     RS_SF(wave_scratch, 0, RS_N_DOFS, 6)
 
 #define RS_GLOBAL_FIELDS(RS_SF)                      \
+    RS_SF(broad_pairs, 1, RS_N_BROAD, 2)             \
     RS_SF(cache_normal, 0, RS_N_PAIRS, 3)            \
     RS_SF(contact_geom_a, 1, RS_N_CONTACTS, 1)       \
     RS_SF(contact_geom_b, 1, RS_N_CONTACTS, 1)       \

@@ -95,8 +95,8 @@ __device__ inline void wave_broadphase(const DModel& m, const DState& loc, const
                 } else {
                     const int idx = first + k++;
                     if (idx < m.BROAD) {
-                        loc.broad_pairs[2 * idx] = ga;
-                        loc.broad_pairs[2 * idx + 1] = gb;
+                        st.broad_pairs[(size_t)(2 * idx) * st.B + b] = ga;
+                        st.broad_pairs[(size_t)(2 * idx + 1) * st.B + b] = gb;
                     }
                 }
             }
@@ -121,8 +121,8 @@ __device__ inline void wave_narrowphase(const DModel& m, const DState& loc, cons
             sink.n = 0;
             int ga = 0, gb = 0, pair = 0;
             if (p < nb) {
-                ga = loc.broad_pairs[2 * p];
-                gb = loc.broad_pairs[2 * p + 1];
+                ga = st.broad_pairs[(size_t)(2 * p) * st.B + b];
+                gb = st.broad_pairs[(size_t)(2 * p + 1) * st.B + b];
                 if (m.geom_type[ga] > m.geom_type[gb]) {
                     int t = ga;
                     ga = gb;
@@ -185,7 +185,8 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     // ConstraintSolver.resolve
     if (!m.disable_constraint) {
         wave_constraints(m, loc, st, b, R, lane);
-        wave_solve(m, loc, st, b, R, lane);
+        if (m.D <= 16) wave_solve<16>(m, loc, st, b, R, lane);
+        else wave_solve<0>(m, loc, st, b, R, lane);
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
     wave_update_acc(m, loc, lane, true);

@@ -1,7 +1,7 @@
 // Wavefront-per-environment execution for the fused substep.
 //
 // One 64-lane wavefront owns one environment for the whole substep. The environment's small fields (links, joints,
-// dofs, geoms, the mass matrix and solver matrices, SAP buffers, broadphase pairs) are staged from the [row][env]
+// dofs, geoms, the mass matrix and solver matrices, SAP buffers) are staged from the [row][env]
 // planes into a private region - LDS when it fits, else an env-major slice of the work field - and written back at
 // the end. Constraint rows, per-row temporaries and per-pair contact slots live in the env-major work field. Lanes
 // split row, pair, contact and matrix-entry loops; code that is inherently sequential (kinematic chains, the SAP
@@ -19,8 +19,7 @@ constexpr int kMaxStateFields = 128;
 // kinds whose fields are staged into the per-environment private region
 __host__ __device__ constexpr bool is_local_kind(int kind) {
     return kind == RS_N_ONE || kind == RS_N_LINKS || kind == RS_N_JOINTS || kind == RS_N_DOFS || kind == RS_N_QS ||
-           kind == RS_N_GEOMS || kind == RS_N_ENTITIES || kind == RS_N_DOFS2 || kind == RS_N_SORT ||
-           kind == RS_N_BROAD;
+           kind == RS_N_GEOMS || kind == RS_N_ENTITIES || kind == RS_N_DOFS2 || kind == RS_N_SORT;
 }
 
 // Launch parameters of the fused kernel; offsets are in floats.
@@ -59,6 +58,27 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
     }
     total = __shfl(x, kWave - 1, kWave);
     return x - v;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// For every q < NQ: acc[q] + v[q](lane l0) + v[q](lane l0 + 1) + ... + v[q](lane l1 - 1), added in lane order (the
+// sequential order of the rows the lanes hold); lane bounds are wave-uniform, the result is the same on every lane.
+// The NQ chains are independent, so their adds interleave.
+template <int NQ>
+__device__ __forceinline__ void lane_seq_sums(float (&acc)[NQ], const float (&v)[NQ], int l0, int l1) {
+    if (l0 == 0 && l1 == kWave) {
+#pragma unroll
+        for (int l = 0; l < kWave; l++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] = acc[q] + readlane_f(v[q], l);
+    } else {
+        for (int l = l0; l < l1; l++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] = acc[q] + readlane_f(v[q], l);
+    }
 }
 
 // acc + f(i0) + f(i0 + 1) + ... + f(i1 - 1), added left to right; up to U terms are loaded before they are added, so

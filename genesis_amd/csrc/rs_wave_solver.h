@@ -311,7 +311,297 @@ __device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const Wo
     wave_sync();
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// Register variants for models with at most KD dofs (instantiated for KD = 16): each lane holds its row's values in registers and
+// every sum over rows is an ordered cross-lane sum (lane_seq_sums), so no row value makes a round trip through memory
+// on the path of a dependent add. Same arithmetic and order as the functions above.
+
+// solver.py:2620-2694
+template <int KD>
+__device__ inline float wave_update_constraint_r(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+                                                 float cost_in) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    LF(prev_cost, 0) = cost_in;
+    // friction-loss terms first (the reference adds them to the cost inside its row loop, before anything else)
+    float cost[1] = {0.f};
+    for (int c = ne; c < nef; c++) {
+        const float f = R.fl(c), r = R.diag(c), ja = R.Jaref(c);
+        const float rf = r * f;
+        const bool ln = ja <= -rf, lp = ja >= rf;
+        float fc = (float)ln * f * (-0.5f * rf - ja);
+        fc = fc + (float)lp * f * (-0.5f * rf + ja);
+        cost[0] = cost[0] + fc;
+    }
+    float gauss = 0.f;
+    for (int d = 0; d < D; d++) {
+        float v = 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
+        gauss = gauss + v;
+        cost[0] = cost[0] + v;
+    }
+    float q[KD];
+#pragma unroll
+    for (int d = 0; d < KD; d++) q[d] = 0.f;
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+        const int c = c0 + lane;
+        float quad[1] = {0.f}, jf[KD];
+        if (c < nc) {
+            R.prev(c) = R.active(c);
+            int act = 1;
+            float ff = 0.f;
+            const float ja = R.Jaref(c);
+            if (ne <= c && c < nef) {
+                const float f = R.fl(c), r = R.diag(c);
+                const float rf = r * f;
+                const bool ln = ja <= -rf, lp = ja >= rf;
+                act = !(ln || lp);
+                ff = (float)ln * f + (float)lp * -f;
+            } else if (nef <= c) {
+                act = ja < 0.f;
+            }
+            R.active(c) = act;
+            const float Dc = R.efc_D(c);
+            const float force = ff + (-ja * Dc * (float)act);
+            R.force(c) = force;
+            quad[0] = 0.5f * (ja * ja * Dc * (float)act);
+#pragma unroll
+            for (int d = 0; d < KD; d++) jf[d] = d < D ? R.jac(c, d) * force : 0.f;
+        } else {
+#pragma unroll
+            for (int d = 0; d < KD; d++) jf[d] = 0.f;
+        }
+        const int n = min(kWave, nc - c0);
+        lane_seq_sums<KD>(q, jf, 0, n);
+        lane_seq_sums<1>(cost, quad, 0, n);
+    }
+#pragma unroll
+    for (int d = 0; d < KD; d++)
+        if (d < D) LF(qfrc_constraint, d) = q[d];
+    LF(gauss, 0) = gauss;
+    wave_sync();
+    return cost[0];
+}
+
+// solver.py:1782-1822, lane-parallel: lane i keeps v_i and the diagonal L[i][i] in registers and owns row i of the
+// factor in LDS. For each flipped row (in row order) and each k, the rotation is computed from v_k and L[k][k]
+// (read from lane k) and applied by the lanes below k.
+template <int KD>
+__device__ inline bool wave_cholesky_incremental_r(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    const bool own = lane < D;
+    float dg = own ? LF(nt_H, lane * D + lane) : 1.f;
+    bool touched = false;
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+        const int cl = c0 + lane;
+        int act = 0;
+        float efd = 0.f;
+        bool flip = false;
+        if (cl < nc) {
+            act = R.active(cl);
+            flip = (act ^ R.prev(cl)) != 0;
+            efd = R.efc_D(cl);
+        }
+        uint64_t flips = wave_ballot(flip);
+        while (flips) {
+            const int l = __builtin_ctzll(flips);
+            flips &= flips - 1;
+            const int c = c0 + l;
+            const float sign = __builtin_amdgcn_readlane(act, l) ? 1.f : -1.f;
+            const float sq = sqrtf(readlane_f(efd, l));
+            float v = own ? R.jac(c, lane) * sq : 0.f;
+            touched = true;
+            for (int k = 0; k < D; k++) {
+                const bool below = own && lane > k;
+                const float hik = below ? LF(nt_H, lane * D + k) : 0.f;
+                const float vk = readlane_f(v, k);
+                if (!(fabsf(vk) > m.eps)) continue;
+                const float Lkk = readlane_f(dg, k);
+                const float t = Lkk * Lkk + sign * vk * vk;
+                if (t < m.eps) return true;
+                const float r = sqrtf(t);
+                const float cc = r / Lkk, cinv = 1.f / cc, ss = vk / Lkk;
+                if (below) {
+                    const float h = (hik + ss * v * sign) * cinv;
+                    LF(nt_H, lane * D + k) = h;
+                    v = v * cc - ss * h;
+                }
+                if (lane == k) dg = r;
+            }
+        }
+    }
+    if (touched && own) LF(nt_H, lane * D + lane) = dg;
+    wave_sync();
+    return false;
+}
+
+// solver.py:2722-2751 + :1893-1910. Forward substitution column by column (lane i subtracts L[i][j] x_j for j = 0, 1,
+// ... in turn - the reference's order for row i); backward substitution row by row in the reference's order, with
+// x_j and L[j][i] read from lane j.
+template <int KD>
+__device__ inline void wave_gradient_r(const DModel& m, const DState& loc, int lane) {
+    const int D = m.D;
+    const bool own = lane < D;
+    float cur = 0.f, hd = 1.f, x = 0.f;
+    if (own) {
+        const float g = LF(Ma, lane) - LF(force, lane) - LF(qfrc_constraint, lane);
+        LF(grad, lane) = g;
+        cur = g;
+        hd = LF(nt_H, lane * D + lane);
+    }
+    for (int j = 0; j < D; j++) {
+        const float hij = (own && lane > j) ? LF(nt_H, lane * D + j) : 0.f;
+        const float xj = readlane_f(cur, j) / readlane_f(hd, j);
+        if (lane == j) x = xj;
+        if (own && lane > j) cur = cur - hij * xj;
+    }
+    for (int ii = 0; ii < D; ii++) {
+        const int i = D - 1 - ii;
+        const float col = (own && lane > i) ? LF(nt_H, lane * D + i) : 0.f;  // lane j holds L[j][i]
+        float c = readlane_f(x, i);
+        for (int j = i + 1; j < D; j++) c = c - readlane_f(col, j) * readlane_f(x, j);
+        c = c / readlane_f(hd, i);
+        if (lane == i) x = c;
+    }
+    if (own) LF(Mgrad, lane) = x;
+    wave_sync();
+}
+
+// solver.py:2034-2152
+template <int KD>
+__device__ inline LsP wave_ls_init_r(const DModel& m, const DState& loc, const WorkRows& R, int lane, int& ls_it) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    for (int i = lane; i < D; i += kWave) {
+        const int e = m.dof_entity[i];
+        float mv = 0.f;
+        for (int j = m.entity_dof_start[e]; j < m.entity_dof_end[e]; j++) mv = mv + LF(mass_mat, i * D + j) * LF(search, j);
+        LF(mv, i) = mv;
+    }
+    float sr[KD];
+#pragma unroll
+    for (int d = 0; d < KD; d++) sr[d] = d < D ? LF(search, d) : 0.f;
+    wave_sync();
+    float g1 = 0.f, g2 = 0.f;
+    for (int d = 0; d < D; d++) {
+        float s = LF(search, d);
+        g1 = g1 + (s * LF(Ma, d) - s * LF(force, d));
+        g2 = g2 + 0.5f * s * LF(mv, d);
+    }
+    const float gauss = LF(gauss, 0);
+    LF(quad_gauss, 0) = gauss;
+    LF(quad_gauss, 1) = g1;
+    LF(quad_gauss, 2) = g2;
+    float t[3] = {gauss, g1, g2}, e[3] = {0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+        const int c = c0 + lane;
+        float f[3] = {0.f, 0.f, 0.f};
+        if (c < nc) {
+            float jv = 0.f;
+#pragma unroll
+            for (int d = 0; d < KD; d++)
+                if (d < D) jv = jv + R.jac(c, d) * sr[d];
+            R.jv(c) = jv;
+            const float ja = R.Jaref(c), Dc = R.efc_D(c);
+            float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+            if (c < ne) {
+            } else if (c < nef) {
+                const float fl = R.fl(c), r = R.diag(c), rf = r * fl;
+                const bool ln = ja <= -rf, lp = ja >= rf;
+                if (ln || lp) {
+                    f0 = (float)ln * fl * (-0.5f * rf - ja) + (float)lp * fl * (-0.5f * rf + ja);
+                    f1 = (float)ln * (-fl * jv) + (float)lp * (fl * jv);
+                    f2 = 0.f;
+                }
+            } else {
+                const float a = (float)(ja < 0.f);
+                f0 = f0 * a;
+                f1 = f1 * a;
+                f2 = f2 * a;
+            }
+            f[0] = f0;
+            f[1] = f1;
+            f[2] = f2;
+        }
+        const int n = min(kWave, nc - c0);
+        lane_seq_sums<3>(e, f, 0, max(0, min(n, ne - c0)));
+        lane_seq_sums<3>(t, f, 0, n);
+    }
+    LF(eq_sum, 0) = e[0];
+    LF(eq_sum, 1) = e[1];
+    LF(eq_sum, 2) = e[2];
+    float h = 2.f * t[2];
+    if (h <= 0.f) h = m.eps;
+    ls_it = 1;
+    wave_sync();
+    return {0.f, t[0], t[1], h};
+}
+
+// solver.py:2155-2355 for N = 1 or 3 candidate steps
+template <int N>
+__device__ inline void wave_ls_eval_r(const DModel& m, const DState& loc, const WorkRows& R, int lane, const float* al,
+                                      LsP* out, int& ls_it) {
+    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
+    const float b0 = LF(quad_gauss, 0) + LF(eq_sum, 0), b1 = LF(quad_gauss, 1) + LF(eq_sum, 1),
+                b2 = LF(quad_gauss, 2) + LF(eq_sum, 2);
+    float T[3 * N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        T[3 * k] = b0;
+        T[3 * k + 1] = b1;
+        T[3 * k + 2] = b2;
+    }
+    float A[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) A[k] = al[k];
+    for (int c0 = (ne / kWave) * kWave; c0 < nc; c0 += kWave) {
+        const int c = c0 + lane;
+        float v[3 * N];
+#pragma unroll
+        for (int q = 0; q < 3 * N; q++) v[q] = 0.f;
+        if (c >= ne && c < nc) {
+            const float ja = R.Jaref(c), jv = R.jv(c), Dc = R.efc_D(c);
+            const float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
+            if (c < nef) {
+                const float f = R.fl(c), r = R.diag(c), rf = r * f;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const float x = ja + A[k] * jv;
+                    const bool ln = x <= -rf, lp = x >= rf;
+                    float a0 = f0, a1 = f1, a2 = f2;
+                    if (ln || lp) {
+                        a0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
+                        a1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
+                        a2 = 0.f;
+                    }
+                    v[3 * k] = a0;
+                    v[3 * k + 1] = a1;
+                    v[3 * k + 2] = a2;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const float a = (float)(ja + A[k] * jv < 0.f);
+                    v[3 * k] = f0 * a;
+                    v[3 * k + 1] = f1 * a;
+                    v[3 * k + 2] = f2 * a;
+                }
+            }
+        }
+        lane_seq_sums<3 * N>(T, v, max(0, ne - c0), min(kWave, nc - c0));
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const float a = A[k];
+        float h = 2.f * T[3 * k + 2];
+        if (h <= 0.f) h = m.eps;
+        out[k] = {a, a * a * T[3 * k + 2] + a * T[3 * k + 1] + T[3 * k], 2.f * a * T[3 * k + 2] + T[3 * k + 1], h};
+    }
+    ls_it += N;
+}
+
 // solver.py:2427-2597 func_linesearch_batch (uniform control flow)
+template <int KD>
 __device__ inline float wave_linesearch(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
     float sn = 0.f;
@@ -326,10 +616,13 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
         LF(ls_it, 0) = 0;
         return 0.f;
     }
-    LsP p0 = wave_ls_init(m, loc, R, lane, ls_it);
+    LsP p0;
+    if constexpr (KD > 0) p0 = wave_ls_init_r<KD>(m, loc, R, lane, ls_it);
+    else p0 = wave_ls_init(m, loc, R, lane, ls_it);
     LsP p1;
     float a = p0.alpha - p0.grad / p0.hess;
-    wave_ls_eval(m, loc, R, lane, &a, 1, &p1, ls_it);
+    if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &a, &p1, ls_it);
+    else wave_ls_eval(m, loc, R, lane, &a, 1, &p1, ls_it);
     if (p0.cost < p1.cost) p1 = p0;
     bool done = false;
     if (fabsf(p1.grad) < gtol) {
@@ -345,7 +638,8 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
             p2 = p1;
             p2update = 1;
             float an = p1.alpha - p1.grad / p1.hess;
-            wave_ls_eval(m, loc, R, lane, &an, 1, &p1, ls_it);
+            if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &an, &p1, ls_it);
+            else wave_ls_eval(m, loc, R, lane, &an, 1, &p1, ls_it);
             if (fabsf(p1.grad) < gtol) {
                 res = p1.alpha;
                 done = true;
@@ -367,7 +661,8 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
         float al[3] = {p1.alpha - p1.grad / p1.hess, p1.alpha, (p1.alpha + p2.alpha) * 0.5f};
         while (ls_it < m.ls_iterations) {
             LsP c[3];
-            wave_ls_eval(m, loc, R, lane, al, 3, c, ls_it);
+            if constexpr (KD > 0) wave_ls_eval_r<3>(m, loc, R, lane, al, c, ls_it);
+            else wave_ls_eval(m, loc, R, lane, al, 3, c, ls_it);
             float n1 = al[0], n2 = al[1];
             bool found = false;
             float ba = 0.f, bc = 0.f;
@@ -412,6 +707,7 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
 }
 
 // solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
+template <int KD>
 __device__ inline void wave_solve(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
                                   int lane) {
     const int D = m.D;
@@ -429,18 +725,21 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
         R.active(c) = 0;  // the first update compares against a cleared previous active set
     }
     wave_sync();
-    float cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
+    float cost;
+    if constexpr (KD > 0) cost = wave_update_constraint_r<KD>(m, loc, R, lane, LF(cost, 0));
+    else cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
     LF(cost, 0) = cost;
     int improved = nc > 0;
     wave_hessian_cholesky(m, loc, R, lane);
-    wave_gradient(m, loc);
+    if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+    else wave_gradient(m, loc);
     for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
     int n_iter = 0;
     if (nc > 0) {
         const float tol = (m.meaninertia * (float)max(1, D)) * m.tolerance;
         for (int it = 0; it < m.iterations; it++) {
             n_iter++;
-            float alpha = wave_linesearch(m, loc, R, lane);
+            float alpha = wave_linesearch<KD>(m, loc, R, lane);
             if (fabsf(alpha) < m.eps) {
                 improved = 0;
             } else {
@@ -451,10 +750,19 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
                 for (int c = lane; c < nc; c += kWave) R.Jaref(c) = R.Jaref(c) + R.jv(c) * alpha;
                 wave_sync();
                 float prev = LF(cost, 0);
-                cost = wave_update_constraint(m, loc, R, lane, prev);
-                LF(cost, 0) = cost;
-                if (wave_cholesky_incremental(m, loc, R, lane)) wave_hessian_cholesky(m, loc, R, lane);
-                wave_gradient(m, loc);
+                bool degenerate;
+                if constexpr (KD > 0) {
+                    cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev);
+                    LF(cost, 0) = cost;
+                    degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane);
+                } else {
+                    cost = wave_update_constraint(m, loc, R, lane, prev);
+                    LF(cost, 0) = cost;
+                    degenerate = wave_cholesky_incremental(m, loc, R, lane);
+                }
+                if (degenerate) wave_hessian_cholesky(m, loc, R, lane);
+                if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+                else wave_gradient(m, loc);
                 float gn = 0.f;
                 for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
                 gn = sqrtf(gn);
