@@ -7,7 +7,8 @@
 // SAP endpoints) come first and their planes are packed back to back, so that one environment's
 // local fields form a single strided sequence: local row k of environment b is element k * n_envs + b of the slab.
 // The fused kernel stages exactly that sequence into its private region. Fields sized by broadphase pairs, pairs,
-// contacts and constraint rows follow (RS_GLOBAL_FIELDS); the last one, wk, is the env-major working set of the fused kernel.
+// contacts and constraint rows follow (RS_GLOBAL_FIELDS), then wk, the env-major working set of the fused kernel, and sched,
+// its cost-ordered environment queues.
 #pragma once
 
 #define RS_LOCAL_FIELDS(RS_SF)               \
@@ -120,6 +121,7 @@
     RS_SF(efc_force, 0, RS_N_CONSTRAINTS, 1)         \
     RS_SF(active, 1, RS_N_CONSTRAINTS, 1)            \
     RS_SF(prev_active, 1, RS_N_CONSTRAINTS, 1)       \
-    RS_SF(wk, 0, RS_N_WORK, 1)
+    RS_SF(wk, 0, RS_N_WORK, 1)                       \
+    RS_SF(sched, 1, RS_N_SCHED, 1)
 
 #define RS_STATE_FIELDS(RS_SF) RS_LOCAL_FIELDS(RS_SF) RS_GLOBAL_FIELDS(RS_SF)

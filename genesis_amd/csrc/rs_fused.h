@@ -185,8 +185,8 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     // ConstraintSolver.resolve
     if (!m.disable_constraint) {
         wave_constraints(m, loc, st, b, R, lane);
-        if (m.D <= 16) wave_solve<16>(m, loc, st, b, R, lane);
-        else wave_solve<0>(m, loc, st, b, R, lane);
+        if (fa.tile_off >= 0) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, base + fa.tile_off);
+        else wave_solve<0>(m, loc, st, b, R, lane, nullptr);
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
     wave_update_acc(m, loc, lane, true);
@@ -202,13 +202,51 @@ __device__ __forceinline__ int xcd_block(int blk, int nblk) {
     return (nblk & 7) == 0 ? (blk & 7) * (nblk >> 3) + (blk >> 3) : blk;
 }
 
-template <bool kLds>
-__global__ void __launch_bounds__(512) k_fused_wave(DModel m, DState st, FusedArgs fa) {
+// flips the schedule parity and empties the queues the coming substep appends to (one thread, before each substep)
+__global__ void k_sched_flip(DState st) {
+    int* ctl = st.sched + (size_t)2 * kSchedClasses * st.B;
+    const int par = (ctl[0] & 1) ^ 1;
+    ctl[0] = par;
+    for (int k = 0; k < kSchedClasses; k++) ctl[1 + (par ^ 1) * kSchedClasses + k] = 0;
+}
+
+// kOcc > 0 asks the compiler for at least kOcc wavefronts per SIMD (fewer registers, more spills); 0 leaves the
+// register budget to the compiler. The autotuner picks between them.
+template <bool kLds, int kOcc>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kOcc > 0 ? kOcc : 1, 8)))
+k_fused_wave(DModel m, DState st, FusedArgs fa) {
     extern __shared__ float lds_env[];
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int b = xcd_block((int)blockIdx.x, (int)gridDim.x) * fa.epb + w;
+    int b = xcd_block((int)blockIdx.x, (int)gridDim.x) * fa.epb + w;
+    int* ctl = nullptr;
+    int par = 0;
+    if (fa.sched) {
+        ctl = st.sched + (size_t)2 * kSchedClasses * st.B;
+        par = ctl[0] & 1;
+        int cnt[kSchedClasses], total = 0;
+        bool valid = true;
+#pragma unroll
+        for (int k = 0; k < kSchedClasses; k++) {
+            cnt[k] = ctl[1 + par * kSchedClasses + k];
+            valid &= cnt[k] >= 0 && cnt[k] <= st.B;
+            total += cnt[k];
+        }
+        int pos = (int)blockIdx.x * fa.epb + w;  // dispatch order
+        if (valid && total == st.B && pos < st.B) {
+            int k = 0;
+            while (k < kSchedClasses - 1 && pos >= cnt[k]) pos -= cnt[k++];
+            const int e = __builtin_amdgcn_readfirstlane(st.sched[(size_t)(par * kSchedClasses + k) * st.B + pos]);
+            if (e >= 0 && e < st.B) b = e;
+        }
+    }
     if (b >= st.B) return;  // whole wavefronts only
     fused_env<kLds>(m, st, fa, b, w, lds_env);
+    if (fa.sched && lane_id() == 0) {
+        const int cls = sched_class(st.n_con[b]);
+        const int q = par ^ 1;
+        const int slot = atomicAdd(&ctl[1 + q * kSchedClasses + cls], 1);
+        if (slot < st.B) st.sched[(size_t)(q * kSchedClasses + cls) * st.B + slot] = b;
+    }
 }
 
 // One batch of a shared launch: a model, its state slab, and the first workgroup that serves it.

@@ -58,6 +58,8 @@ struct rs_model {
     int device = 0;
     FusedArgs fa;       // fused-kernel layout and launch shape
     int use_lds = 0;    // private regions in LDS
+    int schedule = 1;   // cost-ordered environment queues
+    int occupancy = 0;  // 0: compiler's register budget; 4: at least four wavefronts per SIMD
     int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
 };
 
@@ -101,6 +103,7 @@ static int64_t count_of(const rs_model_header& h, int kind) {
         case RS_N_BROAD: return h.max_collision_pairs_broad;
         case RS_N_SORT: return 2 * (int64_t)h.n_geoms;
         case RS_N_WORK: return work_floats(h);
+        case RS_N_SCHED: return sched_rows();
         default: return 0;
     }
 }
@@ -125,7 +128,13 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     }
     fa->n_local = (int)off;
     off = up4(off);
+    fa->tile_off = -1;
+    if (h.n_dofs <= kTileMaxDofs) {
+        fa->tile_off = (int)off;
+        off += up4((int64_t)kTileStride * (h.n_dofs + 2));
+    }
     fa->loc_floats = (int)off;
+    fa->sched = 0;
     const int64_t C = h.len_constraints > 0 ? h.len_constraints : 1, D = h.n_dofs;
     fa->off_loc = 0;
     fa->off_jac = (int)up4(off);
@@ -280,11 +289,19 @@ static int launch_stage(const rs_model* m, const DState& st, int stage, hipStrea
 
 static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     const int epb = m->fa.epb;
+    FusedArgs fa = m->fa;
+    fa.sched = m->schedule && st.B >= sched_rows();
+    if (fa.sched) hipLaunchKernelGGL(k_sched_flip, dim3(1), dim3(1), 0, s, st);
     const dim3 grid((unsigned)((st.B + epb - 1) / epb)), block((unsigned)(kWave * epb));
-    if (m->use_lds)
-        hipLaunchKernelGGL(k_fused_wave<true>, grid, block, (size_t)epb * m->fa.loc_floats * 4, s, m->dm, st, m->fa);
+    const size_t lds = (size_t)epb * fa.loc_floats * 4;
+    if (m->use_lds && m->occupancy == 4)
+        hipLaunchKernelGGL((k_fused_wave<true, 4>), grid, block, lds, s, m->dm, st, fa);
+    else if (m->use_lds)
+        hipLaunchKernelGGL((k_fused_wave<true, 0>), grid, block, lds, s, m->dm, st, fa);
+    else if (m->occupancy == 4)
+        hipLaunchKernelGGL((k_fused_wave<false, 4>), grid, block, 0, s, m->dm, st, fa);
     else
-        hipLaunchKernelGGL(k_fused_wave<false>, grid, block, 0, s, m->dm, st, m->fa);
+        hipLaunchKernelGGL((k_fused_wave<false, 0>), grid, block, 0, s, m->dm, st, fa);
     HIP_OK(hipGetLastError());
     return RS_OK;
 }
@@ -405,6 +422,8 @@ int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
     out->envs_per_block = m->fa.epb;
     out->use_lds = m->use_lds;
     out->lds_bytes_per_env = m->fa.loc_floats * 4;
+    out->schedule = m->schedule;
+    out->occupancy = m->occupancy;
     return RS_OK;
 }
 
@@ -413,6 +432,8 @@ int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     if (int r = check_fused_config(m, c->envs_per_block, c->use_lds != 0)) return r;
     m->fa.epb = c->envs_per_block;
     m->use_lds = c->use_lds != 0;
+    m->schedule = c->schedule != 0;
+    m->occupancy = c->occupancy == 4 ? 4 : 0;
     return RS_OK;
 }
 
@@ -580,6 +601,7 @@ int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* cons
         tab[i].st = make_state(models[i], states[i], n_envs[i]);
         tab[i].fa = models[i]->fa;
         tab[i].fa.epb = epb;
+        tab[i].fa.sched = 0;
         tab[i].first_block = blocks;
         tab[i].n_blocks = (n_envs[i] + epb - 1) / epb;
         blocks += tab[i].n_blocks;

@@ -29,8 +29,26 @@ struct FusedArgs {
     int loc_floats;    // private region per environment (n_local rounded up to 16 bytes)
     int64_t work_stride;  // floats of the work field per environment
     int off_loc, off_jac, off_rowf, off_tmp, off_pc;
+    int tile_off;      // row tile in the private region (kTileStride x (D + 2) floats), -1 when D > kTileMaxDofs
+    int sched;         // 1: map wavefronts to environments through the cost-ordered queues
     int loc_off[kMaxStateFields];  // private-region offset of each state field (local kinds only)
 };
+
+// Row tile: one chunk of 64 constraint rows staged column by column (column k at k * kTileStride), so that the
+// lanes that sum a column over the rows read it from LDS; the odd stride spreads the columns over the banks.
+constexpr int kTileStride = 65;
+constexpr int kTileMaxDofs = 16;
+
+// Schedule: after its substep each environment appends itself to one of kSchedClasses queues by cost (its number of
+// constraint rows); the next substep concatenates the queues, costliest first, and wavefront w takes the w-th
+// environment, so that long environments start early instead of finishing last. The sched field holds the queues of
+// both parities (rows [parity * kSchedClasses + class]) and a control row: [0] the parity of the queues the next
+// substep reads, [1 + parity * kSchedClasses + class] the queue lengths.
+constexpr int kSchedClasses = 4;
+__host__ __device__ constexpr int sched_rows() { return 2 * kSchedClasses + 1; }
+__device__ __forceinline__ int sched_class(int n_con) {
+    return n_con >= 150 ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
+}
 
 // per-row arrays of the constraint store (each n_constraints floats, after the jacobian)
 enum { RW_DIAG = 0, RW_AREF, RW_EFC_D, RW_FL, RW_JAREF, RW_JV, RW_FORCE, RW_ACTIVE, RW_PREV, RW_COUNT };

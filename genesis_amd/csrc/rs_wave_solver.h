@@ -319,7 +319,7 @@ __device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const Wo
 // solver.py:2620-2694
 template <int KD>
 __device__ inline float wave_update_constraint_r(const DModel& m, const DState& loc, const WorkRows& R, int lane,
-                                                 float cost_in) {
+                                                 float cost_in, float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
     LF(prev_cost, 0) = cost_in;
@@ -339,12 +339,10 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const DState& 
         gauss = gauss + v;
         cost[0] = cost[0] + v;
     }
-    float q[KD];
-#pragma unroll
-    for (int d = 0; d < KD; d++) q[d] = 0.f;
+    float q = 0.f;  // lane d < D: qfrc_constraint[d]
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
-        float quad[1] = {0.f}, jf[KD];
+        float quad[1] = {0.f};
         if (c < nc) {
             R.prev(c) = R.active(c);
             int act = 1;
@@ -364,22 +362,96 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const DState& 
             const float force = ff + (-ja * Dc * (float)act);
             R.force(c) = force;
             quad[0] = 0.5f * (ja * ja * Dc * (float)act);
-#pragma unroll
-            for (int d = 0; d < KD; d++) jf[d] = d < D ? R.jac(c, d) * force : 0.f;
-        } else {
-#pragma unroll
-            for (int d = 0; d < KD; d++) jf[d] = 0.f;
+            for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(c, d);
+            tile[D * kTileStride + lane] = force;
         }
         const int n = min(kWave, nc - c0);
-        lane_seq_sums<KD>(q, jf, 0, n);
         lane_seq_sums<1>(cost, quad, 0, n);
+        wave_sync();
+        if (lane < D) {
+            const float* col = tile + lane * kTileStride;
+            const float* fc = tile + D * kTileStride;
+            q = seq_sum(q, 0, n, [&](int l) { return col[l] * fc[l]; });
+        }
+        wave_sync();
     }
-#pragma unroll
-    for (int d = 0; d < KD; d++)
-        if (d < D) LF(qfrc_constraint, d) = q[d];
+    if (lane < D) LF(qfrc_constraint, lane) = q;
     LF(gauss, 0) = gauss;
     wave_sync();
     return cost[0];
+}
+
+// solver.py:1395-1452 + :1595-1621 with each chunk of rows staged in the row tile: one lane per Hessian entry sums
+// its terms over the rows in row order from LDS
+__device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+                                                   float* tile) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    const int nE = D * (D + 1) / 2;
+    const float eps = m.eps;
+    constexpr int kE = (kTileMaxDofs * (kTileMaxDofs + 1) / 2 + kWave - 1) / kWave;  // entries per lane
+    float h[kE];
+#pragma unroll
+    for (int r = 0; r < kE; r++) h[r] = 0.f;
+    for (int c0 = 0; c0 < nc; c0 += kWave) {
+        const int c = c0 + lane;
+        if (c < nc) {
+            for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(c, d);
+            tile[D * kTileStride + lane] = R.efc_D(c);
+            tile[(D + 1) * kTileStride + lane] = (float)R.active(c);
+        }
+        const int n = min(kWave, nc - c0);
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < kE; r++) {
+            const int e = lane + r * kWave;
+            if (e < nE) {
+                int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+                while ((i + 1) * (i + 2) / 2 <= e) i++;
+                while (i * (i + 1) / 2 > e) i--;
+                const int j = e - i * (i + 1) / 2;
+                const float* ci = tile + i * kTileStride;
+                const float* cj = tile + j * kTileStride;
+                const float* cd = tile + D * kTileStride;
+                const float* ca = tile + (D + 1) * kTileStride;
+                float acc = h[r];
+                for (int l = 0; l < n; l++) {
+                    const float ji = ci[l];
+                    if (fabsf(ji) > eps) acc = acc + cj[l] * ji * cd[l] * ca[l];
+                }
+                h[r] = acc;
+            }
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < kE; r++) {
+        const int e = lane + r * kWave;
+        if (e < nE) {
+            int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+            while ((i + 1) * (i + 2) / 2 <= e) i++;
+            while (i * (i + 1) / 2 > e) i--;
+            const int j = e - i * (i + 1) / 2;
+            float v = h[r];
+            if (m.dof_entity[i] == m.dof_entity[j]) v = v + LF(mass_mat, i * D + j);
+            LF(nt_H, i * D + j) = v;
+        }
+    }
+    wave_sync();
+    for (int i = 0; i < D; i++) {
+        float t = LF(nt_H, i * D + i);
+        for (int j = 0; j < i; j++) t = t - LF(nt_H, i * D + j) * LF(nt_H, i * D + j);
+        float dii = sqrtf(fmaxf(t, eps));
+        float inv = 1.f / dii;
+        for (int j = i + 1 + lane; j < D; j += kWave) {
+            float dt = 0.f;
+            for (int k = 0; k < i; k++) dt = dt + LF(nt_H, j * D + k) * LF(nt_H, i * D + k);
+            LF(nt_H, j * D + i) = (LF(nt_H, j * D + i) - dt) * inv;
+        }
+        wave_sync();
+        LF(nt_H, i * D + i) = dii;
+    }
+    wave_sync();
 }
 
 // solver.py:1782-1822, lane-parallel: lane i keeps v_i and the diagonal L[i][i] in registers and owns row i of the
@@ -709,7 +781,7 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
 // solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
 template <int KD>
 __device__ inline void wave_solve(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
-                                  int lane) {
+                                  int lane, float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
     const bool ws = nc > 0 && LF(is_warmstart, 0);
@@ -726,11 +798,12 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
     }
     wave_sync();
     float cost;
-    if constexpr (KD > 0) cost = wave_update_constraint_r<KD>(m, loc, R, lane, LF(cost, 0));
+    if constexpr (KD > 0) cost = wave_update_constraint_r<KD>(m, loc, R, lane, LF(cost, 0), tile);
     else cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
     LF(cost, 0) = cost;
     int improved = nc > 0;
-    wave_hessian_cholesky(m, loc, R, lane);
+    if constexpr (KD > 0) wave_hessian_cholesky_tiled(m, loc, R, lane, tile);
+    else wave_hessian_cholesky(m, loc, R, lane);
     if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
     else wave_gradient(m, loc);
     for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
@@ -752,7 +825,7 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
                 float prev = LF(cost, 0);
                 bool degenerate;
                 if constexpr (KD > 0) {
-                    cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev);
+                    cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev, tile);
                     LF(cost, 0) = cost;
                     degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane);
                 } else {
@@ -760,7 +833,10 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
                     LF(cost, 0) = cost;
                     degenerate = wave_cholesky_incremental(m, loc, R, lane);
                 }
-                if (degenerate) wave_hessian_cholesky(m, loc, R, lane);
+                if (degenerate) {
+                    if constexpr (KD > 0) wave_hessian_cholesky_tiled(m, loc, R, lane, tile);
+                    else wave_hessian_cholesky(m, loc, R, lane);
+                }
                 if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
                 else wave_gradient(m, loc);
                 float gn = 0.f;

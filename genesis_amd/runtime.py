@@ -31,7 +31,7 @@ ERR_INVALID_ACC_NAN = 1 << 3
 
 COUNT_KINDS = [
     "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
-    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK",
+    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK", "SCHED",
 ]
 ENV_MAJOR_KINDS = {"WORK"}
 
@@ -47,7 +47,7 @@ class _ArrayInfo(ctypes.Structure):
 
 class FusedConfig(ctypes.Structure):
     _fields_ = [("envs_per_block", ctypes.c_int32), ("use_lds", ctypes.c_int32),
-                ("lds_bytes_per_env", ctypes.c_int32)]
+                ("lds_bytes_per_env", ctypes.c_int32), ("schedule", ctypes.c_int32), ("occupancy", ctypes.c_int32)]
 
 
 _lib = None
@@ -165,10 +165,10 @@ class DeviceModel:
         c = FusedConfig()
         _check(_lib.rs_fused_config_get(self.handle, ctypes.byref(c)))
         return {"envs_per_block": c.envs_per_block, "use_lds": bool(c.use_lds),
-                "lds_bytes_per_env": c.lds_bytes_per_env}
+                "lds_bytes_per_env": c.lds_bytes_per_env, "schedule": bool(c.schedule), "occupancy": c.occupancy}
 
-    def set_fused_config(self, envs_per_block: int, use_lds: bool):
-        c = FusedConfig(envs_per_block, int(use_lds), 0)
+    def set_fused_config(self, envs_per_block: int, use_lds: bool, schedule: bool = True, occupancy: int = 0):
+        c = FusedConfig(envs_per_block, int(use_lds), 0, int(schedule), int(occupancy))
         _check(_lib.rs_fused_config_set(self.handle, ctypes.byref(c)))
 
 
@@ -274,16 +274,17 @@ def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=Non
     unchanged. The winner is stored in the model and returned with the timings (ms per substep)."""
     dm = slab.dmodel
     if candidates is None:
-        candidates = [(e, lds) for lds in (True, False) for e in (1, 2, 4, 8)]
+        candidates = [(e, lds, sch, occ) for occ in (0, 4) for sch in (True, False) for lds in (True, False)
+                      for e in (1, 2, 4, 8)]
     saved = dm.fused_config
     scratch = StateSlab.__new__(StateSlab)
     scratch.__dict__.update(slab.__dict__)
     scratch.buffer = slab.buffer.clone()
     s = stream or torch.cuda.current_stream()
     timings = {}
-    for epb, lds in candidates:
+    for cand in candidates:
         try:
-            dm.set_fused_config(epb, lds)
+            dm.set_fused_config(*cand)
         except RigidStepError:
             continue
         scratch.buffer.copy_(slab.buffer)
@@ -293,12 +294,17 @@ def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=Non
         scratch.step(steps, RS_MODE_FUSED, s)
         e1.record(s)
         e1.synchronize()
-        timings[(epb, lds)] = e0.elapsed_time(e1) / steps
+        timings[tuple(cand)] = e0.elapsed_time(e1) / steps
     del scratch
     if not timings:
-        dm.set_fused_config(saved["envs_per_block"], saved["use_lds"])
+        dm.set_fused_config(saved["envs_per_block"], saved["use_lds"], saved["schedule"], saved["occupancy"])
         raise RigidStepError("autotune: no runnable fused configuration")
     best = min(timings, key=timings.get)
     dm.set_fused_config(*best)
-    return {"envs_per_block": best[0], "use_lds": best[1], "ms_per_substep": timings[best],
-            "timings": {f"epb{e}_{'lds' if l else 'hbm'}": t for (e, l), t in timings.items()}}
+    def name(c):
+        return "epb{}_{}{}{}".format(c[0], "lds" if c[1] else "hbm", "" if len(c) < 3 or c[2] else "_nosched",
+                                     "_occ4" if len(c) > 3 and c[3] == 4 else "")
+
+    return {"envs_per_block": best[0], "use_lds": best[1], "schedule": best[2] if len(best) > 2 else True,
+            "occupancy": best[3] if len(best) > 3 else 0, "ms_per_substep": timings[best],
+            "timings": {name(c): t for c, t in timings.items()}}

@@ -142,7 +142,9 @@ class Scene:
         else:
             fresh = runtime.StateSlab(self.dmodel, self._B, self.device)
             m = mask.bool()
-            for name, t in slab.fields.items():
+            for (name, _, kind, _), t in zip(slab.directory, slab.fields.values()):
+                if kind in ("WORK", "SCHED"):  # fused-kernel scratch and batch-level schedule: not per-env state
+                    continue
                 t[:, m] = fresh.fields[name][:, m]
             del fresh
         self.t = 0

@@ -73,6 +73,7 @@ enum {
     RS_N_DOFS2, RS_N_GEOMS2, RS_N_PAIRS, RS_N_SUPPORT, RS_N_CONTACTS, RS_N_CONSTRAINTS, RS_N_CONSTRAINTS_DOFS,
     RS_N_BROAD, RS_N_SORT,
     RS_N_WORK,  /* per-environment working set of the fused substep (size from rs_state_field_rows) */
+    RS_N_SCHED, /* fused-substep schedule: 2 x 4 cost-class queues of environment ids and one control row */
     RS_N_KINDS
 };
 
@@ -153,13 +154,17 @@ int rs_reset_caches(const rs_model* model, void* state, int n_envs, const int32_
 int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_t* out, void* stream);
 
 /* Launch configuration of the fused substep for one model: one 64-lane wavefront per environment,
- * envs_per_block wavefronts per workgroup (1..8), and whether the environment's small fields are staged in LDS
- * (use_lds = 1, only when they fit) or in the env-major work field. rs_fused_config_set rejects configurations the
+ * envs_per_block wavefronts per workgroup (1..8), whether the environment's small fields are staged in LDS
+ * (use_lds = 1, only when they fit) or in the env-major work field, and whether environments are started in order of
+ * their previous substep's cost (schedule = 1; every environment's result is the same in any order), and the kernel's
+ * register budget (occupancy 0 or 4). rs_fused_config_set rejects configurations the
  * device cannot run. The model keeps the configuration for every later RS_MODE_FUSED step. */
 typedef struct rs_fused_config {
     int32_t envs_per_block;
     int32_t use_lds;
     int32_t lds_bytes_per_env;  /* out: private-region size of one environment */
+    int32_t schedule;           /* 1: start the environments that were costliest in the previous substep first */
+    int32_t occupancy;          /* 0: compiler's register budget; 4: kernel built for four wavefronts per SIMD */
 } rs_fused_config;
 int rs_fused_config_get(const rs_model* model, rs_fused_config* out);
 int rs_fused_config_set(rs_model* model, const rs_fused_config* config);

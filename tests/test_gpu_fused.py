@@ -86,18 +86,19 @@ def test_fused_launch_shapes_agree():
     want = _twin(rt, dm, ref)
     want.step(5, rt.RS_MODE_PLAIN)
     torch.cuda.synchronize()
-    for lds in (True, False):
-        for epb in (1, 2, 4, 8):
-            try:
-                dm.set_fused_config(epb, lds)
-            except rt.RigidStepError:
-                continue
-            print(f"[fused] launch shape epb={epb} lds={lds}", flush=True)
-            s = _twin(rt, dm, ref)
-            s.step(5, rt.RS_MODE_FUSED)
-            torch.cuda.synchronize()
-            for name in ("qpos", "vel", "acc", "n_contacts"):
-                assert _max_abs(get_dev(s, name), get_dev(want, name)) == 0.0, (epb, lds, name)
+    shapes = [(epb, lds, sched, occ) for occ in (0, 4) for sched in (True, False) for lds in (True, False)
+              for epb in (1, 2, 4, 8)]
+    for epb, lds, sched, occ in shapes:
+        try:
+            dm.set_fused_config(epb, lds, sched, occ)
+        except rt.RigidStepError:
+            continue
+        print(f"[fused] launch shape epb={epb} lds={lds} schedule={sched} occupancy={occ}", flush=True)
+        s = _twin(rt, dm, ref)
+        s.step(5, rt.RS_MODE_FUSED)
+        torch.cuda.synchronize()
+        for name in ("qpos", "vel", "acc", "n_contacts"):
+            assert _max_abs(get_dev(s, name), get_dev(want, name)) == 0.0, (epb, lds, sched, occ, name)
 
 
 @pytest.mark.parametrize("warm", [0, 40])
