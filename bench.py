@@ -135,21 +135,41 @@ def run_path(wl, dm, B, steps, warmup, mode, targets, use_graph, n_lat=200):
     return wall, gpu_ms, lat, nb, slab
 
 
+COPY_BYTES = 1 << 30
+
+
 def hbm_copy_peak_gbps():
-    n = 1 << 28  # 1 GiB of float32
-    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    """Streaming copy of COPY_BYTES through torch's elementwise kernel (b = a * 1): the achievable-bandwidth figure,
+    and the calibration dispatches that the FETCH_SIZE / WRITE_SIZE counter runs scale by."""
+    n = COPY_BYTES // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
-    b.copy_(a)
+    torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(10):
-        b.copy_(a)
+        torch.mul(a, 1.0, out=b)
     e1.record()
     e1.synchronize()
     ms = e0.elapsed_time(e1) / 10
     del a, b
     return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def measured_traffic(kernel, wl, B):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter runs of this workload, if they were
+    taken on the current kernel sources (same source digest); else None."""
+    from genesis_amd.profiling import source_digest
+
+    path = os.path.join(REPO, "profiles", f"{kernel}_{wl.name}{B}.json")
+    if not os.path.exists(path):
+        return None, "no counter profile for this workload"
+    with open(path) as f:
+        fig = json.load(f)
+    if fig.get("source_digest") != source_digest():
+        return None, f"{os.path.basename(path)} was taken on other kernel sources"
+    return fig["traffic"]["bytes_per_launch"], os.path.basename(path)
 
 
 def cpu_baseline(wl, n_envs=128, steps=200):
@@ -188,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fused-config", default=None, metavar="EPB,LDS,SCHED,OCC",
+                    help="fixed fused launch shape (e.g. 1,1,0,0) instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU "
@@ -209,7 +231,10 @@ def main():
     print(f"[bench] {args.workload}: D={model.n_dofs} G={model.n_geoms} pairs={model.n_pairs}; B={B} steps={steps}",
           file=sys.stderr, flush=True)
     tune = None
-    if not args.no_autotune:
+    if args.fused_config:
+        epb, lds, sched, occ = (int(x) for x in args.fused_config.split(","))
+        dm.set_fused_config(epb, bool(lds), bool(sched), occ)
+    elif not args.no_autotune:
         probe = make_slab(wl, dm, B)
         probe.step(20, rt.RS_MODE_FUSED)  # tune on a state with contacts, not the initial one
         tune = rt.autotune_fused(probe, steps=10)
@@ -230,6 +255,7 @@ def main():
     bytes_per_env_step = algorithmic_bytes_per_env_step(model, nb_f)
     kernel_ms = gpu_ms_f / steps  # one fused launch per substep, events on the launch stream
     achieved = bytes_per_env_step * B / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_fused_wave", wl, B)
     out = {
         "metric": "env-steps/s at Franka×4096 (fused vs plain GPU, FP32); p50/p99 step latency",
         "value": value,
@@ -252,7 +278,7 @@ def main():
         "fused_over_plain": (value / plain["value"]) if plain else None,
         "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_env_step": bytes_per_env_step, "mean_overlap_pairs": nb_f,
                      "kernel": "k_fused_wave", "kernel_ms": kernel_ms},
         "errno": errno,

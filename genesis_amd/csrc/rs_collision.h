@@ -13,7 +13,8 @@ constexpr float CCD_EPS = 1e-9f;  // mpr.py:17 (FP32)
 constexpr float CCD_TOL = 1e-6f;
 constexpr int CCD_ITERS = 50;
 
-__device__ inline bool aabb_overlap(const DState& st, int b, int ga, int gb) {
+template <class S>
+__device__ inline bool aabb_overlap(const S& st, int b, int ga, int gb) {
     for (int k = 0; k < 3; k++) {
         if (SF(aabb_max, 3 * ga + k) <= SF(aabb_min, 3 * gb + k)) return false;
         if (SF(aabb_min, 3 * ga + k) >= SF(aabb_max, 3 * gb + k)) return false;
@@ -22,7 +23,8 @@ __device__ inline bool aabb_overlap(const DState& st, int b, int ga, int gb) {
 }
 
 // broadphase.py:74-138 func_collision_clear: reset the contact rows [from, to) of the previous substep
-__device__ inline void clear_contacts(const DState& st, int b, int from, int to) {
+template <class S>
+__device__ inline void clear_contacts(const S& st, int b, int from, int to) {
     for (int i = from; i < to; i++) {
         SF(contact_link_a, i) = SF(contact_link_b, i) = SF(contact_geom_a, i) = SF(contact_geom_b, i) = -1;
         SF(contact_penetration, i) = 0.f;
@@ -34,7 +36,8 @@ __device__ inline void clear_contacts(const DState& st, int b, int from, int to)
 
 // broadphase.py:141-200: first call fills the endpoint buffer in link/geom order; later calls refresh the values of
 // the previous order (warm start)
-__device__ inline void sap_init_endpoints(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void sap_init_endpoints(const DModel& m, const S& st, int b) {
     int ib = 0;
     for (int l = 0; l < m.L; l++)
         for (int g = m.link_geom_start[l]; g < m.link_geom_end[l]; g++) {
@@ -50,7 +53,8 @@ __device__ inline void sap_init_endpoints(const DModel& m, const DState& st, int
 }
 
 // broadphase.py:200-230: insertion sort of the endpoints (stable: equal values keep their previous order)
-__device__ inline void sap_insertion_sort(const DState& st, int b, int n) {
+template <class S>
+__device__ inline void sap_insertion_sort(const S& st, int b, int n) {
     for (int i = 1; i < n; i++) {
         float kv = SF(sort_value, i);
         int km = SF(sort_is_max, i), kg = SF(sort_i_g, i);
@@ -69,7 +73,8 @@ __device__ inline void sap_insertion_sort(const DState& st, int b, int n) {
 
 // broadphase.py:141-396 _func_broad_phase_sap. (st, b) holds the AABBs, sort buffers, active list and pair list;
 // (gs, ge) holds the warm-start normal cache (the same view in the per-stage kernels).
-__device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, const DState& gs, int ge) {
+template <class S>
+__device__ inline void broadphase_sap(const DModel& m, const S& st, int b, const DState& gs, int ge) {
     const int G = m.G;
     if (SF(first_time, 0)) {
         sap_init_endpoints(m, st, b);
@@ -120,7 +125,8 @@ __device__ inline void broadphase_sap(const DModel& m, const DState& st, int b, 
     SF(n_broad, 0) = n_broad;
 }
 
-__device__ inline void dev_broadphase(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_broadphase(const DModel& m, const S& st, int b) {
     clear_contacts(st, b, 0, SF(n_contacts, 0));
     SF(n_contacts, 0) = 0;
     broadphase_sap(m, st, b, st, b);
@@ -140,7 +146,11 @@ __device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
     f3 v = zero3();
     const int start = m.geom_support_start[g];
     for (int i4 = 0; i4 < 4; i4++) {
-        int i = (int)fmodf((i4 & 1) ? ceilf(ii) : floorf(ii), (float)res);
+        // the reference's cell % res; theta lies in [-pi, pi], so the cell lies in [0, res] and the modulo is one
+        // conditional subtraction (a NaN cell stays NaN, as with fmodf)
+        float cell = (i4 & 1) ? ceilf(ii) : floorf(ii);
+        if (cell >= (float)res) cell = cell - (float)res;
+        int i = (int)cell;
         int j;
         if (i4 >> 1) {
             j = (int)clampf(ceilf(jj), 0.f, (float)(res - 1));
@@ -399,7 +409,8 @@ __device__ inline float geom_tolerance(const DModel& m, int ga, int gb, float to
 }
 
 // contact.py:181-222 func_add_contact: write contact row i of (st, b)
-__device__ inline void write_contact(const DModel& m, const DState& st, int b, int i, int ga, int gb, f3 n, f3 p,
+template <class S>
+__device__ inline void write_contact(const DModel& m, const S& st, int b, int i, int ga, int gb, f3 n, f3 p,
                                      float pen, int pair) {
     float fa = m.geom_friction[ga] * 1.f, fb = m.geom_friction[gb] * 1.f;
     SF(contact_geom_a, i) = ga;
@@ -415,7 +426,8 @@ __device__ inline void write_contact(const DModel& m, const DState& st, int b, i
     SF(contact_pair, i) = pair;
 }
 
-__device__ inline void add_contact(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3 p, float pen,
+template <class S>
+__device__ inline void add_contact(const DModel& m, const S& st, int b, int ga, int gb, f3 n, f3 p, float pen,
                                    int pair) {
     int i = SF(n_contacts, 0);
     if (i < m.KMAX) {
@@ -463,7 +475,8 @@ struct SlotSink {
 };
 
 // contact.py:354-412 (principal inertia axes of the link of the smaller geom)
-__device__ inline void contact_axes(const DModel& m, const DState& st, int b, int ga, int gb, f3 n, f3& a0, f3& a1) {
+template <class S>
+__device__ inline void contact_axes(const DModel& m, const S& st, int b, int ga, int gb, f3 n, f3& a0, f3& a1) {
     int g = gb;
     if (m.geom_type[ga] != GT_PLANE) {
         cfptr A = &m.geom_init_aabb[24 * ga + 21];
@@ -583,100 +596,149 @@ __device__ inline bool sphere_capsule(const DModel& m, int ga, int gb, f3 pa, f4
     return col;
 }
 
-// narrowphase.py:520-970 func_convex_convex_contact (MPR code path). Geometry is read from (st, b); the warm-start
-// normal cache lives in (gs, ge); contacts go to the sink.
+// narrowphase.py:520-970 func_convex_convex_contact (MPR code path), in parts so that the fused kernel can run the
+// perturbed detections of one pair on different lanes: convex_setup (pair constants), convex_pose (the poses of
+// detection det: unperturbed for det = 0, rotated about the first contact point for det = 1..4), convex_detect (one
+// detection), convex_first (after detection 0: the first contact, the warm-start cache, the perturbation axes),
+// convex_correct (after detection det >= 1: contact point and normal brought back to the unperturbed frame) and
+// convex_merge (the tolerance test against the pair's earlier contacts, then the append). convex_convex chains them
+// in the reference's order. Each kernel calls convex_detect from one place only, so the MPR code is inlined once.
+// Geometry is read from (st, b); the warm-start normal cache lives in (gs, ge); contacts go to the sink.
+
+// what the detections of a pair need besides the poses
+struct ConvexFirst {
+    f3 n0, p0, ax0, ax1;
+    float tolerance;
+    int pair;
+    bool multi;
+    bool perturb;  // multi-contact pair whose detection 0 collided: detections 1..4 follow
+};
+
+__device__ inline bool convex_setup(const DModel& m, int ga, int gb, ConvexFirst& f) {
+    const int ta = m.geom_type[ga], tb = m.geom_type[gb];
+    f.perturb = false;
+    f.multi = m.enable_multi_contact && ta != GT_SPHERE && ta != GT_ELLIPSOID && tb != GT_SPHERE && tb != GT_ELLIPSOID;
+    f.tolerance = geom_tolerance(m, ga, gb, m.mc_tolerance);
+    f.pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
+    f.n0 = f.p0 = f.ax0 = f.ax1 = zero3();
+    return !(ta == GT_PLANE && tb == GT_BOX);
+}
+
+template <class S>
+__device__ inline void convex_pose(const DModel& m, const S& st, int b, int ga, int gb, const ConvexFirst& f,
+                                   int det, f3& pa, f4q& qa, f3& pb, f4q& qb, f4q& qr) {
+    pa = LD3(geoms_pos, ga);
+    qa = LD4(geoms_quat, ga);
+    pb = LD3(geoms_pos, gb);
+    qb = LD4(geoms_quat, gb);
+    qr = f4q{0.f, 0.f, 0.f, 0.f};
+    if (det > 0) {
+        const f3 axis = (float)(2 * (det % 2) - 1) * f.ax0 + (float)(1 - 2 * ((det / 2) % 2)) * f.ax1;
+        qr = rotvec_quat(m.mc_perturbation * axis, m.eps);
+        rotate_frame(pa, qa, f.p0, qr, pa, qa);
+        rotate_frame(pb, qb, f.p0, qinv(qr), pb, qb);
+    }
+}
+
+// one detection at poses (pa, qa), (pb, qb); detection 0 retries MPR without the warm-start normal
+__device__ inline bool convex_detect(const DModel& m, const DState& gs, int ge, int ga, int gb, int pair, bool first,
+                                     f3 pa, f4q qa, f3 pb, f4q qb, f3& n, f3& cp, float& pen) {
+    const int ta = m.geom_type[ga], tb = m.geom_type[gb];
+    bool col = false;
+    if (ta == GT_CAPSULE && tb == GT_CAPSULE) {
+        col = capsule_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
+    } else if ((ta == GT_SPHERE && tb == GT_CAPSULE) || (ta == GT_CAPSULE && tb == GT_SPHERE)) {
+        col = sphere_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
+    } else if (ta == GT_PLANE) {
+        f3 pd = qrot(mk3(m.geom_data[7 * ga], m.geom_data[7 * ga + 1], m.geom_data[7 * ga + 2]), qa);
+        n = -normalize3(pd);
+        f3 v1 = support_driver(m, n, gb, pb, qb);
+        pen = dot3(n, v1 - pa);
+        cp = v1 - 0.5f * pen * n;
+        col = pen > 0.f;
+    } else {
+        f3 nws = ld3(gs.cache_normal, pair, gs.B, ge);
+        const bool avail = fabsf(nws.x) > m.eps || fabsf(nws.y) > m.eps || fabsf(nws.z) > m.eps;
+#pragma clang loop unroll(disable)
+        for (int i_mpr = 0; i_mpr < 2; i_mpr++) {
+            if (i_mpr == 1) {
+                if (!(first && !col && avail)) break;
+                nws = zero3();
+            }
+            mpr_contact(m, ga, gb, nws, pa, qa, pb, qb, col, n, pen, cp);
+        }
+    }
+    return col;
+}
+
+template <class Sink, class S>
+__device__ inline void convex_first(const DModel& m, const S& st, int b, const DState& gs, int ge, int ga, int gb,
+                                    bool col, f3 n, f3 cp, float pen, Sink& sink, ConvexFirst& f) {
+    f.n0 = n;
+    f.p0 = cp;
+    if (col) {
+        sink.add(ga, gb, n, cp, pen, f.pair);
+        if (f.multi) {
+            contact_axes(m, st, b, ga, gb, n, f.ax0, f.ax1);
+            f.perturb = true;
+        }
+        st3(gs.cache_normal, f.pair, gs.B, ge, n);
+    } else {
+        st3(gs.cache_normal, f.pair, gs.B, ge, zero3());
+    }
+}
+
+__device__ inline void convex_correct(const DModel& m, const ConvexFirst& f, f4q qr, f3& n, f3& cp, float& pen) {
+    const float mcp = m.mc_perturbation;
+    const f3 cpa = qrot((cp - 0.5f * pen * n) - f.p0, qinv(qr)) + f.p0;
+    const f3 cpb = qrot((cp + 0.5f * pen * n) - f.p0, qr) + f.p0;
+    cp = 0.5f * (cpa + cpb);
+    f3 tw = cross3(n, f.n0);
+    tw = mk3(clampf(tw.x, -mcp, mcp), clampf(tw.y, -mcp, mcp), clampf(tw.z, -mcp, mcp));
+    n = n + cross3(tw, n);
+    pen = dot3(n, cpb - cpa);
+}
+
 template <class Sink>
-__device__ inline void convex_convex(const DModel& m, const DState& st, int b, const DState& gs, int ge, int ga, int gb,
+__device__ inline void convex_merge(Sink& sink, int& n_con, int ga, int gb, const ConvexFirst& f, f3 n, f3 cp,
+                                    float pen) {
+    bool rep = false;
+    for (int i = 0; i < n_con; i++)
+        if (!rep && sink.near(i, cp, f.tolerance)) rep = true;
+    if (!rep && pen > -f.tolerance) {
+        pen = fmaxf(pen, 0.f);
+        sink.add(ga, gb, n, cp, pen, f.pair);
+        n_con++;
+    }
+}
+
+template <class Sink, class S>
+__device__ inline void convex_convex(const DModel& m, const S& st, int b, const DState& gs, int ge, int ga, int gb,
                                      Sink& sink) {
-    int ta = m.geom_type[ga], tb = m.geom_type[gb];
-    if (ta == GT_PLANE && tb == GT_BOX) return;
-    const float eps = m.eps, mcp = m.mc_perturbation;
-    bool multi = m.enable_multi_contact && ta != GT_SPHERE && ta != GT_ELLIPSOID && tb != GT_SPHERE && tb != GT_ELLIPSOID;
-    float tolerance = geom_tolerance(m, ga, gb, m.mc_tolerance);
-    f3 pa0 = LD3(geoms_pos, ga), pb0 = LD3(geoms_pos, gb);
-    f4q qa0 = LD4(geoms_quat, ga), qb0 = LD4(geoms_quat, gb);
-    f3 pa = pa0, pb = pb0;
-    f4q qa = qa0, qb = qb0;
-    bool col0 = false, col = false;
-    float pen0 = 0.f, pen = 0.f;
-    f3 n0 = zero3(), p0 = zero3(), n = zero3(), cp = zero3(), ax0 = zero3(), ax1 = zero3();
-    f4q qr{0.f, 0.f, 0.f, 0.f};
-    int n_con = 0;
-    int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
+    ConvexFirst f;
+    if (!convex_setup(m, ga, gb, f)) return;
+    int n_con = 1;
+#pragma clang loop unroll(disable)
     for (int det = 0; det < 5; det++) {
-        if (multi && col0) {
-            f3 axis = (float)(2 * (det % 2) - 1) * ax0 + (float)(1 - 2 * ((det / 2) % 2)) * ax1;
-            qr = rotvec_quat(mcp * axis, eps);
-            rotate_frame(pa0, qa0, p0, qr, pa, qa);
-            rotate_frame(pb0, qb0, p0, qinv(qr), pb, qb);
-        }
-        if ((multi && col0) || det == 0) {
-            if (ta == GT_CAPSULE && tb == GT_CAPSULE) {
-                col = capsule_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
-            } else if ((ta == GT_SPHERE && tb == GT_CAPSULE) || (ta == GT_CAPSULE && tb == GT_SPHERE)) {
-                col = sphere_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
-            } else if (ta == GT_PLANE) {
-                f3 pd = qrot(mk3(m.geom_data[7 * ga], m.geom_data[7 * ga + 1], m.geom_data[7 * ga + 2]), qa);
-                n = -normalize3(pd);
-                f3 v1 = support_driver(m, n, gb, pb, qb);
-                pen = dot3(n, v1 - pa);
-                cp = v1 - 0.5f * pen * n;
-                col = pen > 0.f;
-            } else {
-                bool updated = false;
-                f3 nws = ld3(gs.cache_normal, pair, gs.B, ge);
-                bool avail = fabsf(nws.x) > eps || fabsf(nws.y) > eps || fabsf(nws.z) > eps;
-                for (int i_mpr = 0; i_mpr < 2; i_mpr++) {
-                    if (i_mpr == 1 && det == 0 && !col && avail) {
-                        nws = zero3();
-                        avail = false;
-                        updated = false;
-                    }
-                    if (!updated) {
-                        mpr_contact(m, ga, gb, nws, pa, qa, pb, qb, col, n, pen, cp);
-                        updated = true;
-                    }
-                }
-            }
-        }
+        if (det > 0 && !f.perturb) break;
+        f3 pa, pb;
+        f4q qa, qb, qr;
+        convex_pose(m, st, b, ga, gb, f, det, pa, qa, pb, qb, qr);
+        f3 n = zero3(), cp = zero3();
+        float pen = 0.f;
+        const bool col = convex_detect(m, gs, ge, ga, gb, f.pair, det == 0, pa, qa, pb, qb, n, cp, pen);
         if (det == 0) {
-            col0 = col;
-            n0 = n;
-            pen0 = pen;
-            p0 = cp;
-            if (col0) {
-                sink.add(ga, gb, n0, p0, pen0, pair);
-                if (multi) {
-                    contact_axes(m, st, b, ga, gb, n, ax0, ax1);
-                    n_con = 1;
-                }
-                st3(gs.cache_normal, pair, gs.B, ge, n);
-            } else {
-                st3(gs.cache_normal, pair, gs.B, ge, zero3());
-            }
-        } else if (multi && col) {
-            f3 cpa = qrot((cp - 0.5f * pen * n) - p0, qinv(qr)) + p0;
-            f3 cpb = qrot((cp + 0.5f * pen * n) - p0, qr) + p0;
-            cp = 0.5f * (cpa + cpb);
-            f3 tw = cross3(n, n0);
-            tw = mk3(clampf(tw.x, -mcp, mcp), clampf(tw.y, -mcp, mcp), clampf(tw.z, -mcp, mcp));
-            n = n + cross3(tw, n);
-            pen = dot3(n, cpb - cpa);
-            bool rep = false;
-            for (int i = 0; i < n_con; i++)
-                if (!rep && sink.near(i, cp, tolerance)) rep = true;
-            if (!rep && pen > -tolerance) {
-                pen = fmaxf(pen, 0.f);
-                sink.add(ga, gb, n, cp, pen, pair);
-                n_con++;
-            }
+            convex_first(m, st, b, gs, ge, ga, gb, col, n, cp, pen, sink, f);
+        } else if (col) {
+            convex_correct(m, f, qr, n, cp, pen);
+            convex_merge(sink, n_con, ga, gb, f, n, cp, pen);
         }
     }
 }
 
 // box_contact.py:25-97 func_plane_box_contact
-template <class Sink>
-__device__ inline void plane_box(const DModel& m, const DState& st, int b, int ga, int gb, Sink& sink) {
+template <class Sink, class S>
+__device__ inline void plane_box(const DModel& m, const S& st, int b, int ga, int gb, Sink& sink) {
     f3 pa = LD3(geoms_pos, ga), pb = LD3(geoms_pos, gb);
     f4q qa = LD4(geoms_quat, ga), qb = LD4(geoms_quat, gb);
     int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
@@ -711,7 +773,8 @@ __device__ inline bool convex_pair(const DModel& m, int ta, int tb, int ga, int 
            !(m.box_box_detection && ta == GT_BOX && tb == GT_BOX) && !(ta == GT_PLANE && tb == GT_BOX);
 }
 
-__device__ inline void dev_narrowphase(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_narrowphase(const DModel& m, const S& st, int b) {
     PlaneSink sink{m, st, b};
     int nb = SF(n_broad, 0);
     for (int i = 0; i < nb; i++) {

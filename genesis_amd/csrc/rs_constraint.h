@@ -6,24 +6,24 @@
 
 namespace rs {
 
-#define JAC(c, d) st.jac[((size_t)(c) * m.D + (d)) * st.B + b]
+#define JAC(c, d) FP(st, jac)[((size_t)(c) * m.D + (d)) * st.B + b]
 
 // Row store of the per-stage kernels: the [row][env] planes of the slab.
 struct PlaneRows {
     const DState& st;
     int b, D;
-    __device__ __forceinline__ float& jac(int c, int d) const { return st.jac[((size_t)c * D + d) * st.B + b]; }
-    __device__ __forceinline__ float& diag(int c) const { return st.diag[(size_t)c * st.B + b]; }
-    __device__ __forceinline__ float& aref(int c) const { return st.aref[(size_t)c * st.B + b]; }
-    __device__ __forceinline__ float& efc_D(int c) const { return st.efc_D[(size_t)c * st.B + b]; }
-    __device__ __forceinline__ float& fl(int c) const { return st.efc_frictionloss[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& jac(int c, int d) const { return FP(st, jac)[((size_t)c * D + d) * st.B + b]; }
+    __device__ __forceinline__ float& diag(int c) const { return FP(st, diag)[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& aref(int c) const { return FP(st, aref)[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& efc_D(int c) const { return FP(st, efc_D)[(size_t)c * st.B + b]; }
+    __device__ __forceinline__ float& fl(int c) const { return FP(st, efc_frictionloss)[(size_t)c * st.B + b]; }
 };
 
 // Rows are built from the kinematic state (st, b); contact records are read from (cs, cb).
 
 // solver.py:801-878 func_equality_joint via :881-938; returns the number of rows written from row 0
-template <class R>
-__device__ inline int equality_rows(const DModel& m, const DState& st, int b, const R& rows) {
+template <class R, class S>
+__device__ inline int equality_rows(const DModel& m, const S& st, int b, const R& rows) {
     const int D = m.D;
     int nc = 0;
     for (int e = 0; e < m.NE; e++) {
@@ -58,8 +58,8 @@ __device__ inline int equality_rows(const DModel& m, const DState& st, int b, co
 }
 
 // solver.py:1237-1286 add_frictionloss_constraints; appends from row nc, returns the number of rows added
-template <class R>
-__device__ inline int frictionloss_rows(const DModel& m, const DState& st, int b, const R& rows, int nc) {
+template <class R, class S>
+__device__ inline int frictionloss_rows(const DModel& m, const S& st, int b, const R& rows, int nc) {
     const int D = m.D;
     int nfl = 0;
     for (int l = 0; l < m.L; l++)
@@ -81,8 +81,8 @@ __device__ inline int frictionloss_rows(const DModel& m, const DState& st, int b
 }
 
 // solver.py:580-685 add_collision_constraints for contact ic: rows start + 4 ic .. + 3
-template <class R>
-__device__ inline void contact_rows(const DModel& m, const DState& st, int b, const DState& cs, int cb, const R& rows,
+template <class R, class S>
+__device__ inline void contact_rows(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
                                     int ic, int start) {
     const int D = m.D;
     const size_t CB = (size_t)cs.B;
@@ -128,8 +128,8 @@ __device__ inline void contact_rows(const DModel& m, const DState& st, int b, co
 }
 
 // solver.py:1177-1233 add_joint_limit_constraints; appends from row nc, returns the new row count
-template <class R>
-__device__ inline int limit_rows(const DModel& m, const DState& st, int b, const R& rows, int nc) {
+template <class R, class S>
+__device__ inline int limit_rows(const DModel& m, const S& st, int b, const R& rows, int nc) {
     const int D = m.D;
     for (int l = 0; l < m.L; l++)
         for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
@@ -155,13 +155,15 @@ __device__ inline int limit_rows(const DModel& m, const DState& st, int b, const
     return nc;
 }
 
-__device__ inline void dev_equality_constraints(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_equality_constraints(const DModel& m, const S& st, int b) {
     int nc = equality_rows(m, st, b, PlaneRows{st, b, m.D});
     SF(n_con, 0) = nc;
     SF(n_con_eq, 0) = nc;
 }
 
-__device__ inline void dev_inequality_constraints(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_inequality_constraints(const DModel& m, const S& st, int b) {
     PlaneRows rows{st, b, m.D};
     int nc = SF(n_con, 0);
     int nfl = frictionloss_rows(m, st, b, rows, nc);
@@ -177,7 +179,8 @@ __device__ inline void dev_inequality_constraints(const DModel& m, const DState&
 }
 
 // solver.py:2620-2694
-__device__ inline float dev_update_constraint(const DModel& m, const DState& st, int b, float cost_in) {
+template <class S>
+__device__ inline float dev_update_constraint(const DModel& m, const S& st, int b, float cost_in) {
     const int D = m.D;
     int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
     SF(prev_cost, 0) = cost_in;
@@ -221,7 +224,8 @@ __device__ inline float dev_update_constraint(const DModel& m, const DState& st,
 }
 
 // solver.py:1395-1452 + :1595-1621
-__device__ inline void dev_hessian_cholesky(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_hessian_cholesky(const DModel& m, const S& st, int b) {
     const int D = m.D;
     int nc = SF(n_con, 0);
     for (int i = 0; i < D; i++)
@@ -253,7 +257,8 @@ __device__ inline void dev_hessian_cholesky(const DModel& m, const DState& st, i
 }
 
 // solver.py:1782-1822; returns true when the update degenerates
-__device__ inline bool dev_cholesky_incremental(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline bool dev_cholesky_incremental(const DModel& m, const S& st, int b) {
     const int D = m.D;
     int nc = SF(n_con, 0);
     for (int c = 0; c < nc; c++) {
@@ -278,7 +283,8 @@ __device__ inline bool dev_cholesky_incremental(const DModel& m, const DState& s
 }
 
 // solver.py:2722-2751 + :1893-1910
-__device__ inline void dev_gradient(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_gradient(const DModel& m, const S& st, int b) {
     const int D = m.D;
     for (int d = 0; d < D; d++) SF(grad, d) = SF(Ma, d) - SF(force, d) - SF(qfrc_constraint, d);
     for (int i = 0; i < D; i++) {
@@ -299,7 +305,8 @@ struct LsP {
 };
 
 // solver.py:2034-2152
-__device__ inline LsP dev_ls_init(const DModel& m, const DState& st, int b, int& ls_it) {
+template <class S>
+__device__ inline LsP dev_ls_init(const DModel& m, const S& st, int b, int& ls_it) {
     const int D = m.D;
     int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
     for (int e = 0; e < m.E; e++)
@@ -354,7 +361,8 @@ __device__ inline LsP dev_ls_init(const DModel& m, const DState& st, int b, int&
 }
 
 // solver.py:2155-2355 (n = 1 or 3 candidate steps in one pass over the rows)
-__device__ inline void dev_ls_eval(const DModel& m, const DState& st, int b, const float* al, int n, LsP* out,
+template <class S>
+__device__ inline void dev_ls_eval(const DModel& m, const S& st, int b, const float* al, int n, LsP* out,
                                    int& ls_it) {
     int nc = SF(n_con, 0), ne = SF(n_con_eq, 0), nef = ne + SF(n_con_fl, 0);
     float b0 = SF(quad_gauss, 0) + SF(eq_sum, 0), b1 = SF(quad_gauss, 1) + SF(eq_sum, 1),
@@ -414,7 +422,8 @@ __device__ inline int bracket(LsP& p, const LsP* c, float& next) {
 }
 
 // solver.py:2427-2597 func_linesearch_batch
-__device__ inline float dev_linesearch(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline float dev_linesearch(const DModel& m, const S& st, int b) {
     const int D = m.D;
     float sn = 0.f;
     for (int d = 0; d < D; d++) sn = sn + SF(search, d) * SF(search, d);
@@ -514,7 +523,8 @@ __device__ inline float dev_linesearch(const DModel& m, const DState& st, int b)
 }
 
 // solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
-__device__ inline void dev_solve(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_solve(const DModel& m, const S& st, int b) {
     const int D = m.D;
     int nc = SF(n_con, 0);
     bool ws = nc > 0 && SF(is_warmstart, 0);

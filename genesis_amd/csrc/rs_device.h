@@ -36,6 +36,52 @@ struct DState {
 #undef RS_DS_PTR
 };
 
+// Field identifiers, in directory order, and their element types and count kinds.
+enum FieldId {
+#define RS_FID(name, is_int, kind, width) FID_##name,
+    RS_STATE_FIELDS(RS_FID)
+#undef RS_FID
+    FID_COUNT
+};
+template <int F>
+struct FieldOf;
+#define RS_FOF(name, is_int, kind, width)                                                          \
+    template <>                                                                                     \
+    struct FieldOf<FID_##name> {                                                                    \
+        using T = typename std::conditional<is_int, int32_t, float>::type;                          \
+        static constexpr int count_kind = kind;                                                     \
+        __device__ __forceinline__ static T* get(const DState& s) { return s.name; }               \
+    };
+RS_STATE_FIELDS(RS_FOF)
+#undef RS_FOF
+
+// kinds whose fields the fused kernel keeps in an environment's private region
+__host__ __device__ constexpr bool is_local_kind(int kind) {
+    return kind == RS_N_ONE || kind == RS_N_LINKS || kind == RS_N_JOINTS || kind == RS_N_DOFS || kind == RS_N_QS ||
+           kind == RS_N_GEOMS || kind == RS_N_ENTITIES || kind == RS_N_DOFS2 || kind == RS_N_SORT;
+}
+
+// One environment's private-region fields (fused kernel): field F starts at base + off[F] and has row stride 1. The
+// offsets stay in kernel-argument (constant) memory and every access computes its address from them, so the view
+// holds two uniform values instead of a pointer per field.
+struct LState {
+    static constexpr int B = 1;
+    float* base;
+    const RS_CONST int* off;
+};
+
+template <int F>
+__device__ __forceinline__ typename FieldOf<F>::T* fptr(const DState& s) {
+    return FieldOf<F>::get(s);
+}
+template <int F>
+__device__ __forceinline__ typename FieldOf<F>::T* fptr(const LState& s) {
+    static_assert(is_local_kind(FieldOf<F>::count_kind), "a local view reaches private-region fields only");
+    return (typename FieldOf<F>::T*)(s.base + s.off[F]);
+}
+// pointer to field `field` of state s (a DState: the [row][env] plane; an LState: the private-region copy)
+#define FP(s, field) fptr<FID_##field>(s)
+
 struct f3 {
     float x, y, z;
     __device__ __forceinline__ float& operator[](int i) { return (&x)[i]; }
@@ -239,7 +285,7 @@ __device__ __forceinline__ m33 mmat3(cfptr a, int i) {
 
 // --------------------------------------------------------------------------------- state plane accessors
 // element (row) of a field for environment b: field[row * B + b]
-#define SF(field, row) st.field[(size_t)(row) * st.B + b]
+#define SF(field, row) FP(st, field)[(size_t)(row) * st.B + b]
 
 __device__ __forceinline__ f3 ld3(const float* f, int i, int B, int b) {
     return {f[(size_t)(3 * i) * B + b], f[(size_t)(3 * i + 1) * B + b], f[(size_t)(3 * i + 2) * B + b]};
@@ -269,12 +315,12 @@ __device__ __forceinline__ void st9(float* f, int i, int B, int b, const m33& m)
 #pragma unroll
     for (int k = 0; k < 9; k++) f[(size_t)(9 * i + k) * B + b] = m.a[k / 3][k % 3];
 }
-#define LD3(field, i) ld3(st.field, (i), st.B, b)
-#define ST3(field, i, v) st3(st.field, (i), st.B, b, (v))
-#define LD4(field, i) ld4(st.field, (i), st.B, b)
-#define ST4(field, i, v) st4(st.field, (i), st.B, b, (v))
-#define LD9(field, i) ld9(st.field, (i), st.B, b)
-#define ST9(field, i, v) st9(st.field, (i), st.B, b, (v))
+#define LD3(field, i) ld3(FP(st, field), (i), st.B, b)
+#define ST3(field, i, v) st3(FP(st, field), (i), st.B, b, (v))
+#define LD4(field, i) ld4(FP(st, field), (i), st.B, b)
+#define ST4(field, i, v) st4(FP(st, field), (i), st.B, b, (v))
+#define LD9(field, i) ld9(FP(st, field), (i), st.B, b)
+#define ST9(field, i, v) st9(FP(st, field), (i), st.B, b, (v))
 
 constexpr int CTRL_POSITION = 0, CTRL_VELOCITY = 1, CTRL_FORCE = 2;
 constexpr int JT_FIXED = 0, JT_REVOLUTE = 1, JT_PRISMATIC = 2, JT_SPHERICAL = 3, JT_FREE = 4;

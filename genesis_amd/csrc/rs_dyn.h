@@ -6,7 +6,8 @@
 namespace rs {
 
 // forward_kinematics.py:580-736 func_forward_kinematics_entity
-__device__ inline void dev_forward_kinematics(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_forward_kinematics(const DModel& m, const S& st, int b) {
     for (int l = 0; l < m.L; l++) {
         f3 pos = mvec3(m.link_pos, l);
         f4q quat = mquat(m.link_quat, l);
@@ -66,7 +67,8 @@ __device__ inline void dev_forward_kinematics(const DModel& m, const DState& st,
 }
 
 // forward_kinematics.py:337-576 func_COM_links_entity
-__device__ inline void dev_com_links(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_com_links(const DModel& m, const S& st, int b) {
     for (int e = 0; e < m.E; e++) {
         int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
         // root accumulators live in root_COM (as the weighted sum) and cinr_mass (as mass_sum) until normalised
@@ -142,7 +144,8 @@ __device__ inline void dev_com_links(const DModel& m, const DState& st, int b) {
 }
 
 // forward_kinematics.py:828-864 func_update_geoms_entity
-__device__ inline void dev_update_geoms(const DModel& m, const DState& st, int b, bool force_fixed) {
+template <class S>
+__device__ inline void dev_update_geoms(const DModel& m, const S& st, int b, bool force_fixed) {
     for (int g = 0; g < m.G; g++) {
         if (!force_fixed && m.geom_is_fixed[g]) continue;
         int l = m.geom_link[g];
@@ -155,7 +158,8 @@ __device__ inline void dev_update_geoms(const DModel& m, const DState& st, int b
 }
 
 // forward_kinematics.py:993-1116 func_forward_velocity_entity
-__device__ inline void dev_forward_velocity(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_forward_velocity(const DModel& m, const S& st, int b) {
     for (int l = 0; l < m.L; l++) {
         f3 cv = zero3(), ca = zero3();
         int p = m.link_parent[l];
@@ -203,7 +207,8 @@ __device__ inline void dev_forward_velocity(const DModel& m, const DState& st, i
     }
 }
 
-__device__ inline void dev_kinematics(const DModel& m, const DState& st, int b, bool force_fixed) {
+template <class S>
+__device__ inline void dev_kinematics(const DModel& m, const S& st, int b, bool force_fixed) {
     dev_forward_kinematics(m, st, b);
     dev_com_links(m, st, b);
     dev_update_geoms(m, st, b, force_fixed);
@@ -211,7 +216,8 @@ __device__ inline void dev_kinematics(const DModel& m, const DState& st, int b, 
 }
 
 // forward_dynamics.py:292-541 func_compute_mass_matrix
-__device__ inline void dev_mass_matrix(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_mass_matrix(const DModel& m, const S& st, int b) {
     const int D = m.D;
     for (int l = 0; l < m.L; l++) {
         ST9(crb_inertial, l, LD9(cinr_inertial, l));
@@ -265,7 +271,8 @@ __device__ inline void dev_mass_matrix(const DModel& m, const DState& st, int b)
 }
 
 // forward_dynamics.py:560-602 func_factor_mass (reverse-order L^T D L)
-__device__ inline void dev_factor_mass(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_factor_mass(const DModel& m, const S& st, int b) {
     const int D = m.D;
     for (int e = 0; e < m.E; e++) {
         int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
@@ -287,7 +294,8 @@ __device__ inline void dev_factor_mass(const DModel& m, const DState& st, int b)
 }
 
 // forward_dynamics.py:813-894 func_solve_mass_entity: out = M^-1 vec (vec, out: dof planes)
-__device__ inline void dev_solve_mass(const DModel& m, const DState& st, int b, const float* vec, float* out) {
+template <class S>
+__device__ inline void dev_solve_mass(const DModel& m, const S& st, int b, const float* vec, float* out) {
     const int D = m.D;
     const int B = st.B;
     for (int e = 0; e < m.E; e++) {
@@ -316,7 +324,8 @@ __device__ inline f4q xyz_quat(f3 e) {
 }
 
 // forward_dynamics.py:956-1182 func_torque_and_passive_force
-__device__ inline void dev_torque_passive(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_torque_passive(const DModel& m, const S& st, int b) {
     for (int l = 0; l < m.L; l++) {
         int ds = m.link_dof_start[l], de = m.link_dof_end[l];
         if (de == ds) continue;
@@ -367,7 +376,8 @@ __device__ inline void dev_torque_passive(const DModel& m, const DState& st, int
 }
 
 // forward_dynamics.py:1186-1285 func_update_acc
-__device__ inline void dev_update_acc(const DModel& m, const DState& st, int b, bool cacc) {
+template <class S>
+__device__ inline void dev_update_acc(const DModel& m, const S& st, int b, bool cacc) {
     f3 g = mk3(m.gx, m.gy, m.gz);
     for (int e = 0; e < m.E; e++) {
         float gc = 1.f - m.entity_gravity_compensation[e];
@@ -407,7 +417,8 @@ __device__ inline void dev_update_acc(const DModel& m, const DState& st, int b, 
 }
 
 // forward_dynamics.py:1289-1400 func_update_force, :1428-1486 func_bias_force, :1507-1563 func_compute_qacc
-__device__ inline void dev_rne_and_qacc(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_rne_and_qacc(const DModel& m, const S& st, int b) {
     for (int l = 0; l < m.L; l++) {
         m33 I = LD9(cinr_inertial, l);
         f3 cp = LD3(cinr_pos, l), cdv = LD3(cd_vel, l), cda = LD3(cd_ang, l);
@@ -440,12 +451,13 @@ __device__ inline void dev_rne_and_qacc(const DModel& m, const DState& st, int b
             SF(qf_smooth, d) = f;
         }
     }
-    dev_solve_mass(m, st, b, st.force, st.acc_smooth);
+    dev_solve_mass(m, st, b, FP(st, force), FP(st, acc_smooth));
     for (int d = 0; d < m.D; d++) SF(acc, d) = SF(acc_smooth, d);
 }
 
 // forward_dynamics.py:1567-1707 func_integrate + abd/diff.py:26-55 func_copy_next_to_curr
-__device__ inline void dev_integrate(const DModel& m, const DState& st, int b) {
+template <class S>
+__device__ inline void dev_integrate(const DModel& m, const S& st, int b) {
     const float dt = m.dt;
     for (int d = 0; d < m.D; d++) SF(vel_next, d) = SF(vel, d) + SF(acc, d) * dt;
     for (int l = 0; l < m.L; l++) {
@@ -480,7 +492,8 @@ __device__ inline void dev_integrate(const DModel& m, const DState& st, int b) {
 }
 
 // forward_kinematics.py:1297-1320 kernel_update_geom_aabbs
-__device__ inline void dev_aabb(const DModel& m, const DState& st, int b, int g) {
+template <class S>
+__device__ inline void dev_aabb(const DModel& m, const S& st, int b, int g) {
     f3 p = LD3(geoms_pos, g);
     f4q q = LD4(geoms_quat, g);
     f3 lo = mk3(INFINITY, INFINITY, INFINITY), hi = mk3(-INFINITY, -INFINITY, -INFINITY);

@@ -16,22 +16,25 @@ namespace rs {
 // The endpoints are sorted by rank (rank = number of smaller values + number of equal values earlier in the previous
 // order, i.e. the stable sort that the insertion sort computes); NaN values fall back to the insertion sort itself.
 // In the sequential sweep, the active list when the min endpoint at sorted position p is reached holds exactly the
-// geoms whose min position is below p and whose max position is above p, in min-position order; so lane p walks those
-// candidates itself, counts its pairs, and a prefix sum over positions gives each lane its slice of the pair list.
-// The active_buffer field holds each geom's max position here (it is scratch in both paths).
-__device__ inline void wave_broadphase(const DModel& m, const DState& loc, const DState& st, int b, int lane) {
+// geoms whose min position is below p and whose max position is above p, in min-position order. So the pair list is
+// the list of (a, b), a < b indices into the min endpoints in sorted order, b major and a minor, whose geom a is
+// still active at min endpoint b and which pass the pair filter and the AABB test: for up to 64 geoms each lane
+// takes one (a, b) candidate (index t = b (b - 1) / 2 + a) and a ballot places the survivors in that order; larger
+// models let lane p walk its own candidates. The active_buffer field holds each geom's max position here (it is
+// scratch in both paths).
+__device__ inline void wave_broadphase(const DModel& m, const LState& loc, const DState& st, int b, int lane) {
     const int G = m.G, N = 2 * G;
-    if (loc.first_time[0]) {
+    if (FP(loc, first_time)[0]) {
         sap_init_endpoints(m, loc, 0);
     } else {
         for (int i = lane; i < N; i += kWave) {
-            const int g = loc.sort_i_g[i];
-            loc.sort_value[i] = loc.sort_is_max[i] ? loc.aabb_max[3 * g] : loc.aabb_min[3 * g];
+            const int g = FP(loc, sort_i_g)[i];
+            FP(loc, sort_value)[i] = FP(loc, sort_is_max)[i] ? FP(loc, aabb_max)[3 * g] : FP(loc, aabb_min)[3 * g];
         }
     }
     wave_sync();
     bool nan = false;
-    for (int i = lane; i < N; i += kWave) nan |= isnan(loc.sort_value[i]);
+    for (int i = lane; i < N; i += kWave) nan |= isnan(FP(loc, sort_value)[i]);
     if (wave_ballot(nan) != 0 || N > 2 * kWave) {
         sap_insertion_sort(loc, 0, N);
     } else {
@@ -42,12 +45,12 @@ __device__ inline void wave_broadphase(const DModel& m, const DState& loc, const
             const int i = lane + h * kWave;
             rank[h] = -1;
             if (i < N) {
-                v[h] = loc.sort_value[i];
-                ig[h] = loc.sort_i_g[i];
-                mx[h] = loc.sort_is_max[i];
+                v[h] = FP(loc, sort_value)[i];
+                ig[h] = FP(loc, sort_i_g)[i];
+                mx[h] = FP(loc, sort_is_max)[i];
                 int r = 0;
                 for (int j = 0; j < N; j++) {
-                    const float vj = loc.sort_value[j];
+                    const float vj = FP(loc, sort_value)[j];
                     r += (vj < v[h]) || (vj == v[h] && j < i);
                 }
                 rank[h] = r;
@@ -57,19 +60,70 @@ __device__ inline void wave_broadphase(const DModel& m, const DState& loc, const
 #pragma unroll
         for (int h = 0; h < 2; h++)
             if (rank[h] >= 0) {
-                loc.sort_value[rank[h]] = v[h];
-                loc.sort_i_g[rank[h]] = ig[h];
-                loc.sort_is_max[rank[h]] = mx[h];
+                FP(loc, sort_value)[rank[h]] = v[h];
+                FP(loc, sort_i_g)[rank[h]] = ig[h];
+                FP(loc, sort_is_max)[rank[h]] = mx[h];
             }
     }
     wave_sync();
     for (int p = lane; p < N; p += kWave)
-        if (loc.sort_is_max[p]) loc.active_buffer[loc.sort_i_g[p]] = p;
+        if (FP(loc, sort_is_max)[p]) FP(loc, active_buffer)[FP(loc, sort_i_g)[p]] = p;
     wave_sync();
     int placed = 0;
-    for (int p0 = 0; p0 < N; p0 += kWave) {
+    if (G <= kWave) {
+        // lane k < G: sorted position and geom of the k-th min endpoint (the k-th set bit of the min-endpoint mask)
+        const bool lo_min = lane < N && !FP(loc, sort_is_max)[lane];
+        const bool hi_min = lane + kWave < N && !FP(loc, sort_is_max)[lane + kWave];
+        const uint64_t m_lo = wave_ballot(lo_min), m_hi = wave_ballot(hi_min);
+        const int n_lo = __popcll(m_lo);
+        int pos_k = 0, g_k = 0;
+        if (lane < G) {
+            const uint64_t mask = lane < n_lo ? m_lo : m_hi;
+            const int k = lane < n_lo ? lane : lane - n_lo;
+            int lo = 0, hi = kWave - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (__popcll(mask & ((2ull << mid) - 1)) > k) hi = mid;
+                else lo = mid + 1;
+            }
+            pos_k = (lane < n_lo ? 0 : kWave) + lo;
+            g_k = FP(loc, sort_i_g)[pos_k];
+        }
+        const int T = G * (G - 1) / 2;
+        for (int t0 = 0; t0 < T; t0 += kWave) {
+            const int t = t0 + lane;
+            int bb = (int)((1.f + sqrtf(1.f + 8.f * (float)t)) * 0.5f);
+            while (bb > 1 && bb * (bb - 1) / 2 > t) bb--;
+            while ((bb + 1) * bb / 2 <= t) bb++;
+            const int aa = t - bb * (bb - 1) / 2;
+            const int src_a = t < T ? aa : 0, src_b = t < T ? bb : 0;
+            const int gA = __shfl(g_k, src_a, kWave), gB = __shfl(g_k, src_b, kWave);
+            const int pB = __shfl(pos_k, src_b, kWave);
+            bool emit = false;
+            int ga = 0, gb = 0;
+            if (t < T && FP(loc, active_buffer)[gA] > pB) {
+                ga = min(gA, gB);
+                gb = max(gA, gB);
+                const int pidx = m.collision_pair_idx[ga * G + gb];
+                if (pidx != -1) {
+                    if (aabb_overlap(loc, 0, ga, gb)) emit = true;
+                    else st3(st.cache_normal, pidx, st.B, b, zero3());
+                }
+            }
+            const uint64_t em = wave_ballot(emit);
+            if (emit) {
+                const int idx = placed + __popcll(em & ((1ull << lane) - 1));
+                if (idx < m.BROAD) {
+                    st.broad_pairs[(size_t)(2 * idx) * st.B + b] = ga;
+                    st.broad_pairs[(size_t)(2 * idx + 1) * st.B + b] = gb;
+                }
+            }
+            placed += __popcll(em);
+        }
+    }
+    for (int p0 = 0; G > kWave && p0 < N; p0 += kWave) {
         const int p = p0 + lane;
-        const int gi = (p < N && !loc.sort_is_max[p]) ? loc.sort_i_g[p] : -1;
+        const int gi = (p < N && !FP(loc, sort_is_max)[p]) ? FP(loc, sort_i_g)[p] : -1;
         int cnt = 0;
         for (int pass = 0; pass < 2; pass++) {
             int first = 0;
@@ -80,9 +134,9 @@ __device__ inline void wave_broadphase(const DModel& m, const DState& loc, const
             }
             int k = 0;
             for (int q = 0; gi >= 0 && q < p; q++) {
-                if (loc.sort_is_max[q]) continue;
-                const int g = loc.sort_i_g[q];
-                if (loc.active_buffer[g] < p) continue;  // its max endpoint came first: no longer active
+                if (FP(loc, sort_is_max)[q]) continue;
+                const int g = FP(loc, sort_i_g)[q];
+                if (FP(loc, active_buffer)[g] < p) continue;  // its max endpoint came first: no longer active
                 const int ga = min(g, gi), gb = max(g, gi);
                 const int pidx = m.collision_pair_idx[ga * G + gb];
                 if (pidx == -1) continue;
@@ -102,24 +156,33 @@ __device__ inline void wave_broadphase(const DModel& m, const DState& loc, const
             }
         }
     }
-    if (placed > m.BROAD) loc.errbits[0] = loc.errbits[0] | RS_ERR_OVERFLOW_CANDIDATE_CONTACTS;
-    loc.n_broad[0] = min(placed, m.BROAD);
+    if (placed > m.BROAD) FP(loc, errbits)[0] = FP(loc, errbits)[0] | RS_ERR_OVERFLOW_CANDIDATE_CONTACTS;
+    FP(loc, n_broad)[0] = min(placed, m.BROAD);
     wave_sync();
 }
 
 // narrowphase.py:1988-2189 on one wavefront. Lanes take broadphase pairs; a pair's contacts go to its lane's slot
 // array (capacity n_contacts_per_pair) and are then compacted into the contact rows with a prefix sum, in pair order,
-// convex pairs first and plane-box pairs second - the order in which the sequential narrowphase appends them.
-__device__ inline void wave_narrowphase(const DModel& m, const DState& loc, const DState& st, int b, float* pc,
-                                        int lane) {
-    const int nb = loc.n_broad[0];
+// convex pairs first and plane-box pairs second - the order in which the sequential narrowphase appends them. The
+// four perturbed detections of each multi-contact pair that collided run one per lane (each lane takes a (pair,
+// detection) item); the pair's lane then merges their results in detection order, as the sequential loop does.
+// mc: per-pair records (kMcRec floats per lane), the item map and the item results, in the work field.
+__device__ inline void wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
+                                        float* mc, int lane) {
+    const int nb = FP(loc, n_broad)[0];
     int placed = 0;  // uniform: contacts produced so far
     SlotSink sink{pc + (size_t)lane * m.NCPP * 8, m.NCPP, 0};
+    float* rec = mc;
+    int* imap = (int*)(mc + kWave * kMcRec);
+    float* res = mc + kWave * kMcRec + 4 * kWave;
     for (int pass = 0; pass < 2; pass++) {
         for (int p0 = 0; p0 < nb; p0 += kWave) {
             const int p = p0 + lane;
             sink.n = 0;
             int ga = 0, gb = 0, pair = 0;
+            ConvexFirst f;
+            f.perturb = false;
+            bool conv = false;
             if (p < nb) {
                 ga = st.broad_pairs[(size_t)(2 * p) * st.B + b];
                 gb = st.broad_pairs[(size_t)(2 * p + 1) * st.B + b];
@@ -130,11 +193,82 @@ __device__ inline void wave_narrowphase(const DModel& m, const DState& loc, cons
                 }
                 const int ta = m.geom_type[ga], tb = m.geom_type[gb];
                 pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
-                if (pass == 0) {
-                    if (convex_pair(m, ta, tb, ga, gb)) convex_convex(m, loc, 0, st, b, ga, gb, sink);
-                } else if (ta == GT_PLANE && tb == GT_BOX) {
-                    plane_box(m, loc, 0, ga, gb, sink);
+                if (pass == 0) conv = convex_pair(m, ta, tb, ga, gb) && convex_setup(m, ga, gb, f);
+                else if (ta == GT_PLANE && tb == GT_BOX) plane_box(m, loc, 0, ga, gb, sink);
+            }
+            if (pass == 0) {
+                // round 0: detection 0 of this lane's pair; round 1: the perturbed detections, one (pair, det) item
+                // per lane. Both rounds go through the one detection call below.
+                int n_items = 0, first_item = 0;
+#pragma clang loop unroll(disable)
+                for (int round = 0; round < 2; round++) {
+                    const int n_it = round == 0 ? 1 : (n_items + kWave - 1) / kWave;
+#pragma clang loop unroll(disable)
+                    for (int k = 0; k < n_it; k++) {
+                        const int it = k * kWave + lane;
+                        bool act = round == 0 && conv;
+                        int dga = ga, dgb = gb, det = 0;
+                        ConvexFirst g = f;
+                        if (round == 1 && it < n_items) {
+                            act = true;
+                            const int o = imap[it] >> 2;
+                            det = (imap[it] & 3) + 1;
+                            const float* r = rec + o * kMcRec;
+                            g.n0 = mk3(r[0], r[1], r[2]);
+                            g.p0 = mk3(r[3], r[4], r[5]);
+                            g.ax0 = mk3(r[6], r[7], r[8]);
+                            g.ax1 = mk3(r[9], r[10], r[11]);
+                            g.pair = ((const int*)r)[12];
+                            dga = ((const int*)r)[13];
+                            dgb = ((const int*)r)[14];
+                        }
+                        f3 n = zero3(), cp = zero3();
+                        float pen = 0.f;
+                        f4q qr{0.f, 0.f, 0.f, 0.f};
+                        bool col = false;
+                        if (act) {
+                            f3 pa, pb;
+                            f4q qa, qb;
+                            convex_pose(m, loc, 0, dga, dgb, g, det, pa, qa, pb, qb, qr);
+                            col = convex_detect(m, st, b, dga, dgb, g.pair, det == 0, pa, qa, pb, qb, n, cp, pen);
+                        }
+                        if (round == 0) {
+                            if (act) convex_first(m, loc, 0, st, b, ga, gb, col, n, cp, pen, sink, f);
+                        } else if (act) {
+                            if (col) convex_correct(m, g, qr, n, cp, pen);
+                            float* q = res + (size_t)it * kMcRes;
+                            q[0] = n.x; q[1] = n.y; q[2] = n.z;
+                            q[3] = cp.x; q[4] = cp.y; q[5] = cp.z;
+                            q[6] = pen;
+                            q[7] = col ? 1.f : 0.f;
+                        }
+                    }
+                    if (round == 0) {
+                        if (!wave_ballot(f.perturb)) break;
+                        first_item = wave_excl_scan(f.perturb ? 4 : 0, lane, n_items);
+                        if (f.perturb) {
+                            float* r = rec + lane * kMcRec;
+                            r[0] = f.n0.x; r[1] = f.n0.y; r[2] = f.n0.z;
+                            r[3] = f.p0.x; r[4] = f.p0.y; r[5] = f.p0.z;
+                            r[6] = f.ax0.x; r[7] = f.ax0.y; r[8] = f.ax0.z;
+                            r[9] = f.ax1.x; r[10] = f.ax1.y; r[11] = f.ax1.z;
+                            ((int*)r)[12] = f.pair;
+                            ((int*)r)[13] = ga;
+                            ((int*)r)[14] = gb;
+                            for (int j = 0; j < 4; j++) imap[first_item + j] = lane * 4 + j;
+                        }
+                    }
+                    wave_sync();
                 }
+                if (f.perturb) {
+                    int n_con = 1;
+                    for (int j = 0; j < 4; j++) {
+                        const float* q = res + (size_t)(first_item + j) * kMcRes;
+                        if (q[7] != 0.f)
+                            convex_merge(sink, n_con, ga, gb, f, mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]), q[6]);
+                    }
+                }
+                wave_sync();
             }
             int total;
             const int first = placed + wave_excl_scan(sink.n, lane, total);
@@ -148,8 +282,8 @@ __device__ inline void wave_narrowphase(const DModel& m, const DState& loc, cons
             placed += total;
         }
     }
-    if (placed > m.KMAX) loc.errbits[0] = loc.errbits[0] | RS_ERR_OVERFLOW_COLLISION_PAIRS;
-    loc.n_contacts[0] = min(placed, m.KMAX);
+    if (placed > m.KMAX) FP(loc, errbits)[0] = FP(loc, errbits)[0] | RS_ERR_OVERFLOW_COLLISION_PAIRS;
+    FP(loc, n_contacts)[0] = min(placed, m.KMAX);
     wave_sync();
 }
 
@@ -160,7 +294,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     const int lane = lane_id();
     float* wk = st.wk + (size_t)b * fa.work_stride;
     float* base = kLds ? lds_env + (size_t)w * fa.loc_floats : wk + fa.off_loc;
-    const DState loc = make_local<kLds>(st, base, fa);
+    const LState loc = make_local(base, fa);
     const WorkRows R{wk + fa.off_jac, wk + fa.off_rowf, wk + fa.off_tmp, m.CMAX, m.D};
 
     stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
@@ -176,10 +310,10 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
         for (int g = lane; g < m.G; g += kWave) dev_aabb(m, loc, 0, g);
         wave_sync();
         if (m.P > 0) {
-            const int n_old = loc.n_contacts[0];
+            const int n_old = FP(loc, n_contacts)[0];
             for (int i = lane; i < n_old; i += kWave) clear_contacts(st, b, i, i + 1);
             wave_broadphase(m, loc, st, b, lane);
-            wave_narrowphase(m, loc, st, b, wk + fa.off_pc, lane);
+            wave_narrowphase(m, loc, st, b, wk + fa.off_pc, wk + fa.off_mc, lane);
         }
     }
     // ConstraintSolver.resolve

@@ -131,7 +131,7 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->tile_off = -1;
     if (h.n_dofs <= kTileMaxDofs) {
         fa->tile_off = (int)off;
-        off += up4((int64_t)kTileStride * (h.n_dofs + 2));
+        off += up4((int64_t)kTileStride * tile_rows(h.n_dofs));
     }
     fa->loc_floats = (int)off;
     fa->sched = 0;
@@ -141,7 +141,8 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->off_rowf = (int)(fa->off_jac + up4(C * D));
     fa->off_tmp = (int)(fa->off_rowf + RW_COUNT * up4(C));
     fa->off_pc = (int)(fa->off_tmp + kTmpArrays * up4(C));
-    int64_t total = fa->off_pc + (int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8;
+    fa->off_mc = (int)(fa->off_pc + up4((int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8));
+    int64_t total = fa->off_mc + kMcFloats;
     fa->work_stride = (total + 63) / 64 * 64;
 }
 

@@ -16,11 +16,6 @@ namespace rs {
 constexpr int kWave = 64;
 constexpr int kMaxStateFields = 128;
 
-// kinds whose fields are staged into the per-environment private region
-__host__ __device__ constexpr bool is_local_kind(int kind) {
-    return kind == RS_N_ONE || kind == RS_N_LINKS || kind == RS_N_JOINTS || kind == RS_N_DOFS || kind == RS_N_QS ||
-           kind == RS_N_GEOMS || kind == RS_N_ENTITIES || kind == RS_N_DOFS2 || kind == RS_N_SORT;
-}
 
 // Launch parameters of the fused kernel; offsets are in floats.
 struct FusedArgs {
@@ -28,8 +23,8 @@ struct FusedArgs {
     int n_local;       // local rows staged per environment (the packed local planes)
     int loc_floats;    // private region per environment (n_local rounded up to 16 bytes)
     int64_t work_stride;  // floats of the work field per environment
-    int off_loc, off_jac, off_rowf, off_tmp, off_pc;
-    int tile_off;      // row tile in the private region (kTileStride x (D + 2) floats), -1 when D > kTileMaxDofs
+    int off_loc, off_jac, off_rowf, off_tmp, off_pc, off_mc;
+    int tile_off;      // row tile in the private region (kTileStride x tile_rows(D) floats), -1 when D > kTileMaxDofs
     int sched;         // 1: map wavefronts to environments through the cost-ordered queues
     int loc_off[kMaxStateFields];  // private-region offset of each state field (local kinds only)
 };
@@ -38,6 +33,10 @@ struct FusedArgs {
 // lanes that sum a column over the rows read it from LDS; the odd stride spreads the columns over the banks.
 constexpr int kTileStride = 65;
 constexpr int kTileMaxDofs = 16;
+constexpr int kTileMinRows = 12;  // the ordered sums of the line search use up to 9 rows
+__host__ __device__ constexpr int tile_rows(int n_dofs) {
+    return n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;
+}
 
 // Schedule: after its substep each environment appends itself to one of kSchedClasses queues by cost (its number of
 // constraint rows); the next substep concatenates the queues, costliest first, and wavefront w takes the w-th
@@ -49,6 +48,11 @@ __host__ __device__ constexpr int sched_rows() { return 2 * kSchedClasses + 1; }
 __device__ __forceinline__ int sched_class(int n_con) {
     return n_con >= 150 ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
 }
+
+// perturbed multi-contact detections of the fused narrowphase: per-pair record and per-detection result (floats)
+constexpr int kMcRec = 16;
+constexpr int kMcRes = 8;
+constexpr int kMcFloats = kWave * kMcRec + 4 * kWave + 4 * kWave * kMcRes;
 
 // per-row arrays of the constraint store (each n_constraints floats, after the jacobian)
 enum { RW_DIAG = 0, RW_AREF, RW_EFC_D, RW_FL, RW_JAREF, RW_JV, RW_FORCE, RW_ACTIVE, RW_PREV, RW_COUNT };
@@ -82,23 +86,6 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// For every q < NQ: acc[q] + v[q](lane l0) + v[q](lane l0 + 1) + ... + v[q](lane l1 - 1), added in lane order (the
-// sequential order of the rows the lanes hold); lane bounds are wave-uniform, the result is the same on every lane.
-// The NQ chains are independent, so their adds interleave.
-template <int NQ>
-__device__ __forceinline__ void lane_seq_sums(float (&acc)[NQ], const float (&v)[NQ], int l0, int l1) {
-    if (l0 == 0 && l1 == kWave) {
-#pragma unroll
-        for (int l = 0; l < kWave; l++)
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] = acc[q] + readlane_f(v[q], l);
-    } else {
-        for (int l = l0; l < l1; l++)
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] = acc[q] + readlane_f(v[q], l);
-    }
-}
-
 // acc + f(i0) + f(i0 + 1) + ... + f(i1 - 1), added left to right; up to U terms are loaded before they are added, so
 // that the loads of a batch overlap instead of each add waiting for its own load
 template <int U = 8, class F>
@@ -115,18 +102,32 @@ __device__ __forceinline__ float seq_sum(float acc, int i0, int i1, F f) {
     return acc;
 }
 
+// Ordered sums through the row tile. Lane l writes its row values v[q] to column l of tile row q; lane q (q < NQ) then
+// adds row q's columns l0, l0 + 1, ..., l1 - 1 in that order to its own accumulator acc: the additions of a sequential
+// loop over the rows, with the NQ chains running side by side in NQ lanes instead of as NQ x 64 broadcasts. The row
+// stride kTileStride keeps both the lane-wise writes and the row-wise reads free of bank conflicts. Bounds are
+// wave-uniform; the accumulators stay in lanes 0..NQ-1 (read them back with readlane_f).
+template <int NQ>
+__device__ __forceinline__ void tile_seq_sums(float& acc, const float (&v)[NQ], int l0, int l1, float* tile, int lane) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) tile[q * kTileStride + lane] = v[q];
+    wave_sync();
+    if (lane < NQ) {
+        const float* row = tile + lane * kTileStride;
+        if (l0 == 0 && l1 == kWave) {
+#pragma unroll
+            for (int l = 0; l < kWave; l++) acc = acc + row[l];
+        } else {
+            acc = seq_sum(acc, l0, l1, [&](int l) { return row[l]; });
+        }
+    }
+    wave_sync();
+}
+
 // ---------------------------------------------------------------------------------------- private region
-template <bool kLds>
-__device__ __forceinline__ DState make_local(const DState& st, float* base, const FusedArgs& fa) {
-    DState l = st;
-    l.B = 1;
-    int i = 0;
-#define RS_X(name, is_int, kind, width)                                                           \
-    if constexpr (is_local_kind(kind)) l.name = (decltype(l.name))(base + fa.loc_off[i]);       \
-    i++;
-    RS_STATE_FIELDS(RS_X)
-#undef RS_X
-    return l;
+// the local view of the environment whose private region starts at base
+__device__ __forceinline__ LState make_local(float* base, const FusedArgs& fa) {
+    return LState{base, (const RS_CONST int*)fa.loc_off};
 }
 
 // Copy environment b's local rows between the packed local planes (which start at the slab base, the first local

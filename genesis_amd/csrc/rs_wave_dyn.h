@@ -9,10 +9,10 @@
 
 namespace rs {
 
-#define LF(field, row) loc.field[(row)]
+#define LF(field, row) FP(loc, field)[(row)]
 
 // forward_dynamics.py:292-541 func_compute_mass_matrix
-__device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_mass_matrix(const DModel& m, const LState& loc, int lane) {
     const int D = m.D, L = m.L;
     for (int t = lane; t < L * 13; t += kWave) {
         const int l = t / 13, k = t - 13 * (t / 13);
@@ -23,7 +23,7 @@ __device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int 
     wave_sync();
     // subtree accumulation, leaves to root: lane k owns component k of every link
     if (lane < 13) {
-        float* f = lane < 9 ? loc.crb_inertial : (lane < 12 ? loc.crb_pos : loc.crb_mass);
+        float* f = lane < 9 ? FP(loc, crb_inertial) : (lane < 12 ? FP(loc, crb_pos) : FP(loc, crb_mass));
         const int w = lane < 9 ? 9 : (lane < 12 ? 3 : 1), c = lane < 9 ? lane : (lane < 12 ? lane - 9 : 0);
         for (int e = 0; e < m.E; e++) {
             const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
@@ -37,10 +37,10 @@ __device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int 
     for (int d = lane; d < D; d += kWave) {
         const int l = m.dof_link[d];
         f3 fa, fv;
-        inertial_mul(ld3(loc.crb_pos, l, 1, 0), ld9(loc.crb_inertial, l, 1, 0), LF(crb_mass, l),
-                     ld3(loc.cdof_vel, d, 1, 0), ld3(loc.cdof_ang, d, 1, 0), fa, fv);
-        st3(loc.f_ang, d, 1, 0, fa);
-        st3(loc.f_vel, d, 1, 0, fv);
+        inertial_mul(ld3(FP(loc, crb_pos), l, 1, 0), ld9(FP(loc, crb_inertial), l, 1, 0), LF(crb_mass, l),
+                     ld3(FP(loc, cdof_vel), d, 1, 0), ld3(FP(loc, cdof_ang), d, 1, 0), fa, fv);
+        st3(FP(loc, f_ang), d, 1, 0, fa);
+        st3(FP(loc, f_vel), d, 1, 0, fv);
     }
     wave_sync();
     for (int t = lane; t < D * D; t += kWave) {
@@ -48,8 +48,8 @@ __device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int 
         const int e = m.dof_entity[i];
         if (m.dof_entity[j] != e) continue;
         const int a = i >= j ? i : j, c = i >= j ? j : i;  // the lower-triangle entry (a, c) carries the value
-        float v = (dot3(ld3(loc.f_ang, a, 1, 0), ld3(loc.cdof_ang, c, 1, 0)) +
-                   dot3(ld3(loc.f_vel, a, 1, 0), ld3(loc.cdof_vel, c, 1, 0))) *
+        float v = (dot3(ld3(FP(loc, f_ang), a, 1, 0), ld3(FP(loc, cdof_ang), c, 1, 0)) +
+                   dot3(ld3(FP(loc, f_vel), a, 1, 0), ld3(FP(loc, cdof_vel), c, 1, 0))) *
                   m.mass_parent_mask[a * D + c];
         if (i == j) {
             v = v + m.dof_armature[i];
@@ -65,7 +65,7 @@ __device__ inline void wave_mass_matrix(const DModel& m, const DState& loc, int 
 
 // forward_dynamics.py:560-602 func_factor_mass: for each pivot i (last to first), the rows j < i are updated by
 // separate lanes; row i's entries are read before any lane rescales them
-__device__ inline void wave_factor_mass(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_factor_mass(const DModel& m, const LState& loc, int lane) {
     const int D = m.D;
     for (int e = 0; e < m.E; e++) {
         const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e], n = d1 - d0;
@@ -109,7 +109,7 @@ __device__ inline void wave_factor_mass(const DModel& m, const DState& loc, int 
 }
 
 // forward_dynamics.py:956-1182 func_torque_and_passive_force, one lane per link (then per dof)
-__device__ inline void wave_torque_passive(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_torque_passive(const DModel& m, const LState& loc, int lane) {
     for (int l = lane; l < m.L; l += kWave) {
         const int ds = m.link_dof_start[l], de = m.link_dof_end[l];
         if (de == ds) continue;
@@ -161,13 +161,13 @@ __device__ inline void wave_torque_passive(const DModel& m, const DState& loc, i
 
 // forward_dynamics.py:1186-1285 func_update_acc: lane k < 6 owns component k of (cdd_vel, cdd_ang) and, with cacc,
 // lane 6 + k of (cacc_lin, cacc_ang), walking the links in order
-__device__ inline void wave_update_acc(const DModel& m, const DState& loc, int lane, bool cacc) {
+__device__ inline void wave_update_acc(const DModel& m, const LState& loc, int lane, bool cacc) {
     const int nlanes = cacc ? 12 : 6;
     if (lane < nlanes) {
         const int c = lane % 3, grp = lane / 3;  // 0: cdd_vel, 1: cdd_ang, 2: cacc_lin, 3: cacc_ang
-        float* out = grp == 0 ? loc.cdd_vel : grp == 1 ? loc.cdd_ang : grp == 2 ? loc.cacc_lin : loc.cacc_ang;
-        const float* dd = (grp == 0 || grp == 2) ? loc.cdofd_vel : loc.cdofd_ang;
-        const float* dq = (grp == 0 || grp == 2) ? loc.cdof_vel : loc.cdof_ang;
+        float* out = grp == 0 ? FP(loc, cdd_vel) : grp == 1 ? FP(loc, cdd_ang) : grp == 2 ? FP(loc, cacc_lin) : FP(loc, cacc_ang);
+        const float* dd = (grp == 0 || grp == 2) ? FP(loc, cdofd_vel) : FP(loc, cdofd_ang);
+        const float* dq = (grp == 0 || grp == 2) ? FP(loc, cdof_vel) : FP(loc, cdof_ang);
         const float g = c == 0 ? m.gx : (c == 1 ? m.gy : m.gz);
         for (int e = 0; e < m.E; e++) {
             const float gc = 1.f - m.entity_gravity_compensation[e];
@@ -187,21 +187,21 @@ __device__ inline void wave_update_acc(const DModel& m, const DState& loc, int l
 }
 
 // forward_dynamics.py:1289-1400 func_update_force, :1428-1486 func_bias_force, :1507-1563 func_compute_qacc
-__device__ inline void wave_rne_and_qacc(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_rne_and_qacc(const DModel& m, const LState& loc, int lane) {
     for (int l = lane; l < m.L; l += kWave) {
-        m33 I = ld9(loc.cinr_inertial, l, 1, 0);
-        f3 cp = ld3(loc.cinr_pos, l, 1, 0), cdv = ld3(loc.cd_vel, l, 1, 0), cda = ld3(loc.cd_ang, l, 1, 0);
+        m33 I = ld9(FP(loc, cinr_inertial), l, 1, 0);
+        f3 cp = ld3(FP(loc, cinr_pos), l, 1, 0), cdv = ld3(FP(loc, cd_vel), l, 1, 0), cda = ld3(FP(loc, cd_ang), l, 1, 0);
         float cm = LF(cinr_mass, l);
         f3 f1a, f1v, f2a, f2v, f3a, f3v;
-        inertial_mul(cp, I, cm, ld3(loc.cdd_vel, l, 1, 0), ld3(loc.cdd_ang, l, 1, 0), f1a, f1v);
+        inertial_mul(cp, I, cm, ld3(FP(loc, cdd_vel), l, 1, 0), ld3(FP(loc, cdd_ang), l, 1, 0), f1a, f1v);
         inertial_mul(cp, I, cm, cdv, cda, f2a, f2v);
         motion_x_force(cda, cdv, f2a, f2v, f3a, f3v);
-        st3(loc.cfrc_vel, l, 1, 0, f1v + f3v + zero3() + zero3());
-        st3(loc.cfrc_ang, l, 1, 0, f1a + f3a + zero3() + zero3());
+        st3(FP(loc, cfrc_vel), l, 1, 0, f1v + f3v + zero3() + zero3());
+        st3(FP(loc, cfrc_ang), l, 1, 0, f1a + f3a + zero3() + zero3());
     }
     wave_sync();
     if (lane < 6) {
-        float* f = lane < 3 ? loc.cfrc_vel : loc.cfrc_ang;
+        float* f = lane < 3 ? FP(loc, cfrc_vel) : FP(loc, cfrc_ang);
         const int c = lane % 3;
         for (int e = 0; e < m.E; e++) {
             const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
@@ -214,30 +214,30 @@ __device__ inline void wave_rne_and_qacc(const DModel& m, const DState& loc, int
     wave_sync();
     for (int d = lane; d < m.D; d += kWave) {
         const int l = m.dof_link[d];
-        f3 fa = ld3(loc.cfrc_ang, l, 1, 0), fv = ld3(loc.cfrc_vel, l, 1, 0);
-        float bias = dot3(ld3(loc.cdof_ang, d, 1, 0), fa) + dot3(ld3(loc.cdof_vel, d, 1, 0), fv);
+        f3 fa = ld3(FP(loc, cfrc_ang), l, 1, 0), fv = ld3(FP(loc, cfrc_vel), l, 1, 0);
+        float bias = dot3(ld3(FP(loc, cdof_ang), d, 1, 0), fa) + dot3(ld3(FP(loc, cdof_vel), d, 1, 0), fv);
         LF(qf_bias, d) = bias;
         float f = LF(qf_passive, d) - bias + LF(qf_applied, d);
         LF(force, d) = f;
         LF(qf_smooth, d) = f;
     }
     wave_sync();
-    dev_solve_mass(m, loc, 0, loc.force, loc.acc_smooth);
+    dev_solve_mass(m, loc, 0, FP(loc, force), FP(loc, acc_smooth));
     for (int d = 0; d < m.D; d++) LF(acc, d) = LF(acc_smooth, d);
 }
 
 // forward_kinematics.py:337-576 func_COM_links_entity
-__device__ inline void wave_com_links(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_com_links(const DModel& m, const LState& loc, int lane) {
     for (int e = 0; e < m.E; e++) {
         const int l0 = m.entity_link_start[e], l1 = m.entity_link_end[e];
         for (int l = l0 + lane; l < l1; l += kWave) {
             f3 ip;
             f4q iq;
-            tr_pose(mvec3(m.link_inertial_pos, l) + zero3(), mquat(m.link_inertial_quat, l), ld3(loc.links_pos, l, 1, 0),
-                    ld4(loc.links_quat, l, 1, 0), ip, iq);
-            st3(loc.i_pos, l, 1, 0, ip);
-            st4(loc.i_quat, l, 1, 0, iq);
-            st3(loc.root_COM, l, 1, 0, zero3());
+            tr_pose(mvec3(m.link_inertial_pos, l) + zero3(), mquat(m.link_inertial_quat, l), ld3(FP(loc, links_pos), l, 1, 0),
+                    ld4(FP(loc, links_quat), l, 1, 0), ip, iq);
+            st3(FP(loc, i_pos), l, 1, 0, ip);
+            st4(FP(loc, i_quat), l, 1, 0, iq);
+            st3(FP(loc, root_COM), l, 1, 0, zero3());
             LF(crb_mass, l) = 0.f;
         }
         wave_sync();
@@ -255,55 +255,55 @@ __device__ inline void wave_com_links(const DModel& m, const DState& loc, int la
             const int r = m.link_root[l];
             if (l == r) {
                 float ms = LF(crb_mass, l);
-                if (ms > m.eps) st3(loc.root_COM, l, 1, 0, ld3(loc.root_COM, l, 1, 0) / ms);
-                else st3(loc.root_COM, l, 1, 0, ld3(loc.i_pos, r, 1, 0));
+                if (ms > m.eps) st3(FP(loc, root_COM), l, 1, 0, ld3(FP(loc, root_COM), l, 1, 0) / ms);
+                else st3(FP(loc, root_COM), l, 1, 0, ld3(FP(loc, i_pos), r, 1, 0));
             }
         }
         wave_sync();
         for (int l = l0 + lane; l < l1; l += kWave) {
-            const f3 rc = ld3(loc.root_COM, m.link_root[l], 1, 0);
-            f3 ipos = ld3(loc.i_pos, l, 1, 0) - rc;
-            st3(loc.i_pos, l, 1, 0, ipos);
+            const f3 rc = ld3(FP(loc, root_COM), m.link_root[l], 1, 0);
+            f3 ipos = ld3(FP(loc, i_pos), l, 1, 0) - rc;
+            st3(FP(loc, i_pos), l, 1, 0, ipos);
             float mass = m.link_inertial_mass[l] + 0.f;
             m33 ci;
             f3 cp;
-            inertia_offset(mmat3(m.link_inertial_i, l), mass, ipos, ld4(loc.i_quat, l, 1, 0), m.eps, ci, cp);
-            st9(loc.cinr_inertial, l, 1, 0, ci);
-            st3(loc.cinr_pos, l, 1, 0, cp);
+            inertia_offset(mmat3(m.link_inertial_i, l), mass, ipos, ld4(FP(loc, i_quat), l, 1, 0), m.eps, ci, cp);
+            st9(FP(loc, cinr_inertial), l, 1, 0, ci);
+            st3(FP(loc, cinr_pos), l, 1, 0, cp);
             LF(cinr_mass, l) = mass;
         }
         wave_sync();
         for (int l = l0 + lane; l < l1; l += kWave) {
-            if (m.link_root[l] != l) st3(loc.root_COM, l, 1, 0, ld3(loc.root_COM, m.link_root[l], 1, 0));
+            if (m.link_root[l] != l) st3(FP(loc, root_COM), l, 1, 0, ld3(FP(loc, root_COM), m.link_root[l], 1, 0));
         }
         for (int j = lane; j < m.J; j += kWave) {
             const int l = m.joint_link[j];
             if (l < l0 || l >= l1 || m.link_dof_end[l] == m.link_dof_start[l]) continue;
-            f3 off = ld3(loc.root_COM, m.link_root[l], 1, 0) - ld3(loc.xanchor, j, 1, 0);
+            f3 off = ld3(FP(loc, root_COM), m.link_root[l], 1, 0) - ld3(FP(loc, xanchor), j, 1, 0);
             const int jt = m.joint_type[j], ds = m.joint_dof_start[j];
-            f3 xa = ld3(loc.xaxis, j, 1, 0);
+            f3 xa = ld3(FP(loc, xaxis), j, 1, 0);
             if (jt == JT_REVOLUTE) {
-                st3(loc.cdof_ang, ds, 1, 0, xa);
-                st3(loc.cdof_vel, ds, 1, 0, cross3(xa, off));
+                st3(FP(loc, cdof_ang), ds, 1, 0, xa);
+                st3(FP(loc, cdof_vel), ds, 1, 0, cross3(xa, off));
             } else if (jt == JT_PRISMATIC) {
-                st3(loc.cdof_ang, ds, 1, 0, zero3());
-                st3(loc.cdof_vel, ds, 1, 0, xa);
+                st3(FP(loc, cdof_ang), ds, 1, 0, zero3());
+                st3(FP(loc, cdof_vel), ds, 1, 0, xa);
             } else if (jt == JT_SPHERICAL || jt == JT_FREE) {
                 int base = ds;
                 if (jt == JT_FREE) {
                     for (int k = 0; k < 3; k++) {
                         f3 e3 = zero3();
                         e3[k] = 1.f;
-                        st3(loc.cdof_ang, ds + k, 1, 0, zero3());
-                        st3(loc.cdof_vel, ds + k, 1, 0, e3);
+                        st3(FP(loc, cdof_ang), ds + k, 1, 0, zero3());
+                        st3(FP(loc, cdof_vel), ds + k, 1, 0, e3);
                     }
                     base = ds + 3;
                 }
-                m33 R = quat_R(ld4(loc.links_quat, l, 1, 0), m.eps);
+                m33 R = quat_R(ld4(FP(loc, links_quat), l, 1, 0), m.eps);
                 for (int k = 0; k < 3; k++) {
                     f3 row = mk3(R.a[0][k], R.a[1][k], R.a[2][k]);
-                    st3(loc.cdof_ang, base + k, 1, 0, row);
-                    st3(loc.cdof_vel, base + k, 1, 0, cross3(row, off));
+                    st3(FP(loc, cdof_ang), base + k, 1, 0, row);
+                    st3(FP(loc, cdof_vel), base + k, 1, 0, cross3(row, off));
                 }
             }
         }
@@ -315,13 +315,13 @@ __device__ inline void wave_com_links(const DModel& m, const DState& loc, int la
 // (cd_vel, cd_ang) and records, for every dof, the (cd_ang, cd_vel) that its cdofd is taken against (the
 // velocity accumulated before the dof's joint - after the translational dofs for a free joint's rotations); then one
 // lane per dof forms cdofd.
-__device__ inline void wave_forward_velocity(const DModel& m, const DState& loc, int lane) {
-    float* pre = loc.wave_scratch;  // [d * 6 + (0..2: ang, 3..5: vel)]
+__device__ inline void wave_forward_velocity(const DModel& m, const LState& loc, int lane) {
+    float* pre = FP(loc, wave_scratch);  // [d * 6 + (0..2: ang, 3..5: vel)]
     if (lane < 6) {
         const int c = lane % 3;
         const bool ang = lane >= 3;
-        float* out = ang ? loc.cd_ang : loc.cd_vel;
-        const float* cd = ang ? loc.cdof_ang : loc.cdof_vel;
+        float* out = ang ? FP(loc, cd_ang) : FP(loc, cd_vel);
+        const float* cd = ang ? FP(loc, cdof_ang) : FP(loc, cdof_vel);
         const int slot = ang ? c : 3 + c;
         for (int l = 0; l < m.L; l++) {
             const int p = m.link_parent[l];
@@ -348,30 +348,30 @@ __device__ inline void wave_forward_velocity(const DModel& m, const DState& loc,
         f3 oa = zero3(), ov = zero3();
         if (!(m.joint_type[j] == JT_FREE && d < m.joint_dof_start[j] + 3)) {
             f3 ca = mk3(pre[6 * d], pre[6 * d + 1], pre[6 * d + 2]), cv = mk3(pre[6 * d + 3], pre[6 * d + 4], pre[6 * d + 5]);
-            motion_x_motion(ca, cv, ld3(loc.cdof_ang, d, 1, 0), ld3(loc.cdof_vel, d, 1, 0), oa, ov);
+            motion_x_motion(ca, cv, ld3(FP(loc, cdof_ang), d, 1, 0), ld3(FP(loc, cdof_vel), d, 1, 0), oa, ov);
         }
-        st3(loc.cdofd_ang, d, 1, 0, oa);
-        st3(loc.cdofd_vel, d, 1, 0, ov);
+        st3(FP(loc, cdofd_ang), d, 1, 0, oa);
+        st3(FP(loc, cdofd_vel), d, 1, 0, ov);
     }
     wave_sync();
 }
 
 // forward_kinematics.py:828-864 func_update_geoms_entity, one lane per geom
-__device__ inline void wave_update_geoms(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_update_geoms(const DModel& m, const LState& loc, int lane) {
     for (int g = lane; g < m.G; g += kWave) {
         if (m.geom_is_fixed[g]) continue;
         const int l = m.geom_link[g];
         f3 p;
         f4q q;
-        tr_pose(mvec3(m.geom_pos, g), mquat(m.geom_quat, g), ld3(loc.links_pos, l, 1, 0), ld4(loc.links_quat, l, 1, 0), p,
+        tr_pose(mvec3(m.geom_pos, g), mquat(m.geom_quat, g), ld3(FP(loc, links_pos), l, 1, 0), ld4(FP(loc, links_quat), l, 1, 0), p,
                 q);
-        st3(loc.geoms_pos, g, 1, 0, p);
-        st4(loc.geoms_quat, g, 1, 0, q);
+        st3(FP(loc, geoms_pos), g, 1, 0, p);
+        st4(FP(loc, geoms_quat), g, 1, 0, q);
     }
 }
 
 // forward kinematics (the chain walk runs on every lane), then COM, geoms and velocities split across lanes
-__device__ inline void wave_kinematics(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_kinematics(const DModel& m, const LState& loc, int lane) {
     dev_forward_kinematics(m, loc, 0);
     wave_sync();
     wave_com_links(m, loc, lane);

@@ -29,10 +29,10 @@ struct WorkRows {
     __device__ __forceinline__ float& tmp(int k, int c) const { return tp[(size_t)k * cap + c]; }
 };
 
-#define LF(field, row) loc.field[(row)]
+#define LF(field, row) FP(loc, field)[(row)]
 
 // solver.py:580-1286: equality, friction-loss and limit rows uniformly (few rows), contact rows one lane per contact
-__device__ inline void wave_constraints(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
+__device__ inline void wave_constraints(const DModel& m, const LState& loc, const DState& st, int b, const WorkRows& R,
                                         int lane) {
     int nc = equality_rows(m, loc, 0, R);
     const int neq = nc;
@@ -51,7 +51,7 @@ __device__ inline void wave_constraints(const DModel& m, const DState& loc, cons
 }
 
 // solver.py:2620-2694 (same arithmetic as dev_update_constraint)
-__device__ inline float wave_update_constraint(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+__device__ inline float wave_update_constraint(const DModel& m, const LState& loc, const WorkRows& R, int lane,
                                                float cost_in) {
     const int D = m.D;
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
@@ -96,7 +96,7 @@ __device__ inline float wave_update_constraint(const DModel& m, const DState& lo
 }
 
 // solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
-__device__ inline void wave_hessian_cholesky(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+__device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
     const int nE = D * (D + 1) / 2;
@@ -148,7 +148,7 @@ __device__ inline void wave_hessian_cholesky(const DModel& m, const DState& loc,
 
 // solver.py:1782-1822: rank-1 updates of the factor for rows whose active flag flipped, in row order;
 // returns true (uniformly) when an update degenerates
-__device__ inline bool wave_cholesky_incremental(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+__device__ inline bool wave_cholesky_incremental(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
     for (int c0 = 0; c0 < nc; c0 += kWave) {
@@ -184,7 +184,7 @@ __device__ inline bool wave_cholesky_incremental(const DModel& m, const DState& 
 }
 
 // solver.py:2722-2751 + :1893-1910 (uniform: D is small; the order of every sum is the reference's)
-__device__ inline void wave_gradient(const DModel& m, const DState& loc) {
+__device__ inline void wave_gradient(const DModel& m, const LState& loc) {
     const int D = m.D;
     for (int d = 0; d < D; d++) LF(grad, d) = LF(Ma, d) - LF(force, d) - LF(qfrc_constraint, d);
     for (int i = 0; i < D; i++) {
@@ -201,7 +201,7 @@ __device__ inline void wave_gradient(const DModel& m, const DState& loc) {
 }
 
 // solver.py:2034-2152 func_ls_init_and_eval_p0_opt
-__device__ inline LsP wave_ls_init(const DModel& m, const DState& loc, const WorkRows& R, int lane, int& ls_it) {
+__device__ inline LsP wave_ls_init(const DModel& m, const LState& loc, const WorkRows& R, int lane, int& ls_it) {
     const int D = m.D;
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
     for (int i = lane; i < D; i += kWave) {
@@ -265,7 +265,7 @@ __device__ inline LsP wave_ls_init(const DModel& m, const DState& loc, const Wor
 }
 
 // solver.py:2155-2355 (n = 1 or 3 candidate steps in one pass over the rows)
-__device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const WorkRows& R, int lane, const float* al,
+__device__ inline void wave_ls_eval(const DModel& m, const LState& loc, const WorkRows& R, int lane, const float* al,
                                     int n, LsP* out, int& ls_it) {
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
     for (int c = ne + lane; c < nc; c += kWave) {
@@ -312,13 +312,14 @@ __device__ inline void wave_ls_eval(const DModel& m, const DState& loc, const Wo
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Register variants for models with at most KD dofs (instantiated for KD = 16): each lane holds its row's values in registers and
-// every sum over rows is an ordered cross-lane sum (lane_seq_sums), so no row value makes a round trip through memory
-// on the path of a dependent add. Same arithmetic and order as the functions above.
+// Variants for models with at most KD dofs (instantiated for KD = 16), which have a row tile in the private region:
+// each lane computes its row's terms in registers and stages them in the tile, and every sum over rows runs in one
+// lane per output, in row order (tile_seq_sums), so many ordered sums proceed side by side. Same arithmetic and order
+// as the functions above.
 
 // solver.py:2620-2694
 template <int KD>
-__device__ inline float wave_update_constraint_r(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+__device__ inline float wave_update_constraint_r(const DModel& m, const LState& loc, const WorkRows& R, int lane,
                                                  float cost_in, float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
@@ -339,10 +340,12 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const DState& 
         gauss = gauss + v;
         cost[0] = cost[0] + v;
     }
-    float q = 0.f;  // lane d < D: qfrc_constraint[d]
+    // lane d < D sums qfrc_constraint[d] = sum_c jac[c][d] force[c] (tile rows d and D + 1), lane D the cost's
+    // quadratic terms (tile row D, times the ones of row D + 2, which leaves them exact), both in row order
+    float q = lane == D ? cost[0] : 0.f;
+    tile[(D + 2) * kTileStride + lane] = 1.f;
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
-        float quad[1] = {0.f};
         if (c < nc) {
             R.prev(c) = R.active(c);
             int act = 1;
@@ -361,29 +364,29 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const DState& 
             const float Dc = R.efc_D(c);
             const float force = ff + (-ja * Dc * (float)act);
             R.force(c) = force;
-            quad[0] = 0.5f * (ja * ja * Dc * (float)act);
             for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(c, d);
-            tile[D * kTileStride + lane] = force;
+            tile[D * kTileStride + lane] = 0.5f * (ja * ja * Dc * (float)act);
+            tile[(D + 1) * kTileStride + lane] = force;
         }
         const int n = min(kWave, nc - c0);
-        lane_seq_sums<1>(cost, quad, 0, n);
         wave_sync();
-        if (lane < D) {
+        if (lane <= D) {
             const float* col = tile + lane * kTileStride;
-            const float* fc = tile + D * kTileStride;
+            const float* fc = tile + (lane < D ? D + 1 : D + 2) * kTileStride;
             q = seq_sum(q, 0, n, [&](int l) { return col[l] * fc[l]; });
         }
         wave_sync();
     }
     if (lane < D) LF(qfrc_constraint, lane) = q;
     LF(gauss, 0) = gauss;
+    cost[0] = readlane_f(q, D);
     wave_sync();
     return cost[0];
 }
 
 // solver.py:1395-1452 + :1595-1621 with each chunk of rows staged in the row tile: one lane per Hessian entry sums
 // its terms over the rows in row order from LDS
-__device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const DState& loc, const WorkRows& R, int lane,
+__device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState& loc, const WorkRows& R, int lane,
                                                    float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
@@ -454,55 +457,80 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const DState
     wave_sync();
 }
 
-// solver.py:1782-1822, lane-parallel: lane i keeps v_i and the diagonal L[i][i] in registers and owns row i of the
-// factor in LDS. For each flipped row (in row order) and each k, the rotation is computed from v_k and L[k][k]
-// (read from lane k) and applied by the lanes below k.
+// solver.py:1782-1822 with the rank-1 updates of consecutive flipped rows pipelined. Update u (the u-th flipped row in
+// row order) applies its rotation for column k at step start(u) + k, and start times strictly increase, so the updates
+// in flight at one step work on different columns of the factor, while every element of the factor still receives the
+// updates in row order, each with the reference's arithmetic. The lanes form S = min(64 / D, D) groups of D lanes;
+// group s carries the update in slot s (lane i of the group holds element i of its working vector), and slots take
+// updates round robin, at most one new update per step. Rows are scanned 64 at a time; the scaled jacobian rows
+// jac[c] * sqrt(efc_D[c]) of a chunk's flipped rows are staged in the row tile, so that starting an update reads LDS
+// only. An update that degenerates makes the whole call return true (uniformly): the caller refactors from scratch,
+// as the reference does once any update has degenerated.
 template <int KD>
-__device__ inline bool wave_cholesky_incremental_r(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+__device__ inline bool wave_cholesky_incremental_r(const DModel& m, const LState& loc, const WorkRows& R, int lane,
+                                                   float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
-    const bool own = lane < D;
-    float dg = own ? LF(nt_H, lane * D + lane) : 1.f;
-    bool touched = false;
-    for (int c0 = 0; c0 < nc; c0 += kWave) {
-        const int cl = c0 + lane;
-        int act = 0;
-        float efd = 0.f;
-        bool flip = false;
-        if (cl < nc) {
-            act = R.active(cl);
-            flip = (act ^ R.prev(cl)) != 0;
-            efd = R.efc_D(cl);
+    if (D <= 0) return false;
+    const int S = min(kWave / D, D);
+    const int s = lane / D, i = lane - s * D;
+    const bool inslot = s < S;
+    const int first = (inslot ? s : 0) * D;  // first lane of this lane's group
+    int k = D;                                // column the group's update works on next; D when the slot is idle
+    float sign = 1.f, v = 0.f;
+    int c0 = -kWave, act = 0;
+    uint64_t flips = 0;
+    int next = 0, busy = 0;
+    for (;;) {
+        while (!flips && c0 + kWave < nc) {
+            c0 += kWave;
+            const int cl = c0 + lane;
+            bool flip = false;
+            wave_sync();  // earlier readers of the tile are done
+            if (cl < nc) {
+                act = R.active(cl);
+                flip = (act ^ R.prev(cl)) != 0;
+                if (flip) {
+                    const float sq = sqrtf(R.efc_D(cl));
+                    for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(cl, d) * sq;
+                }
+            }
+            flips = wave_ballot(flip);
+            wave_sync();
         }
-        uint64_t flips = wave_ballot(flip);
-        while (flips) {
+        if (flips && busy < S) {  // slots finish in the order they start: slot `next` is the idle one
             const int l = __builtin_ctzll(flips);
             flips &= flips - 1;
-            const int c = c0 + l;
-            const float sign = __builtin_amdgcn_readlane(act, l) ? 1.f : -1.f;
-            const float sq = sqrtf(readlane_f(efd, l));
-            float v = own ? R.jac(c, lane) * sq : 0.f;
-            touched = true;
-            for (int k = 0; k < D; k++) {
-                const bool below = own && lane > k;
-                const float hik = below ? LF(nt_H, lane * D + k) : 0.f;
-                const float vk = readlane_f(v, k);
-                if (!(fabsf(vk) > m.eps)) continue;
-                const float Lkk = readlane_f(dg, k);
-                const float t = Lkk * Lkk + sign * vk * vk;
-                if (t < m.eps) return true;
-                const float r = sqrtf(t);
-                const float cc = r / Lkk, cinv = 1.f / cc, ss = vk / Lkk;
-                if (below) {
-                    const float h = (hik + ss * v * sign) * cinv;
-                    LF(nt_H, lane * D + k) = h;
-                    v = v * cc - ss * h;
-                }
-                if (lane == k) dg = r;
+            const float sg = __builtin_amdgcn_readlane(act, l) ? 1.f : -1.f;
+            if (s == next) {
+                k = 0;
+                sign = sg;
+                v = tile[i * kTileStride + l];
             }
+            next = next + 1 == S ? 0 : next + 1;
         }
+        const bool run = inslot && k < D;
+        if (!wave_ballot(run)) break;  // nothing in flight and no flipped row left
+        const int kk = run ? k : 0;
+        const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute((first + kk) << 2, __float_as_int(v)));
+        const float Lkk = LF(nt_H, kk * D + kk);
+        const bool below = run && i > kk;
+        const float hik = below ? LF(nt_H, i * D + kk) : 0.f;
+        const bool rot = run && fabsf(vk) > m.eps;
+        const float t = Lkk * Lkk + sign * vk * vk;
+        if (wave_ballot(rot && t < m.eps)) return true;
+        const float r = sqrtf(t);
+        const float cc = r / Lkk, cinv = 1.f / cc, ss = vk / Lkk;
+        if (rot && below) {
+            const float h = (hik + ss * v * sign) * cinv;
+            LF(nt_H, i * D + kk) = h;
+            v = v * cc - ss * h;
+        }
+        if (rot && i == kk) LF(nt_H, kk * D + kk) = r;
+        if (run) k++;
+        busy = __popcll(wave_ballot(inslot && i == 0 && k < D));
+        wave_sync();
     }
-    if (touched && own) LF(nt_H, lane * D + lane) = dg;
     wave_sync();
     return false;
 }
@@ -511,7 +539,7 @@ __device__ inline bool wave_cholesky_incremental_r(const DModel& m, const DState
 // ... in turn - the reference's order for row i); backward substitution row by row in the reference's order, with
 // x_j and L[j][i] read from lane j.
 template <int KD>
-__device__ inline void wave_gradient_r(const DModel& m, const DState& loc, int lane) {
+__device__ inline void wave_gradient_r(const DModel& m, const LState& loc, int lane) {
     const int D = m.D;
     const bool own = lane < D;
     float cur = 0.f, hd = 1.f, x = 0.f;
@@ -541,7 +569,8 @@ __device__ inline void wave_gradient_r(const DModel& m, const DState& loc, int l
 
 // solver.py:2034-2152
 template <int KD>
-__device__ inline LsP wave_ls_init_r(const DModel& m, const DState& loc, const WorkRows& R, int lane, int& ls_it) {
+__device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const WorkRows& R, int lane, int& ls_it,
+                                     float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
     for (int i = lane; i < D; i += kWave) {
@@ -564,10 +593,11 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const DState& loc, const W
     LF(quad_gauss, 0) = gauss;
     LF(quad_gauss, 1) = g1;
     LF(quad_gauss, 2) = g2;
-    float t[3] = {gauss, g1, g2}, e[3] = {0.f, 0.f, 0.f};
+    // lanes 0-2 sum t[q] = (gauss, g1, g2)[q] + f[q] over all rows, lanes 3-5 e[q] over the equality rows, in row
+    // order, from tile row q
+    float acc = lane == 0 ? gauss : lane == 1 ? g1 : lane == 2 ? g2 : 0.f;
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
-        float f[3] = {0.f, 0.f, 0.f};
         if (c < nc) {
             float jv = 0.f;
 #pragma unroll
@@ -591,17 +621,23 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const DState& loc, const W
                 f1 = f1 * a;
                 f2 = f2 * a;
             }
-            f[0] = f0;
-            f[1] = f1;
-            f[2] = f2;
+            tile[lane] = f0;
+            tile[kTileStride + lane] = f1;
+            tile[2 * kTileStride + lane] = f2;
         }
         const int n = min(kWave, nc - c0);
-        lane_seq_sums<3>(e, f, 0, max(0, min(n, ne - c0)));
-        lane_seq_sums<3>(t, f, 0, n);
+        wave_sync();
+        if (lane < 6) {
+            const float* row = tile + (lane < 3 ? lane : lane - 3) * kTileStride;
+            const int l1 = lane < 3 ? n : max(0, min(n, ne - c0));
+            acc = seq_sum(acc, 0, l1, [&](int l) { return row[l]; });
+        }
+        wave_sync();
     }
-    LF(eq_sum, 0) = e[0];
-    LF(eq_sum, 1) = e[1];
-    LF(eq_sum, 2) = e[2];
+    const float t[3] = {readlane_f(acc, 0), readlane_f(acc, 1), readlane_f(acc, 2)};
+    LF(eq_sum, 0) = readlane_f(acc, 3);
+    LF(eq_sum, 1) = readlane_f(acc, 4);
+    LF(eq_sum, 2) = readlane_f(acc, 5);
     float h = 2.f * t[2];
     if (h <= 0.f) h = m.eps;
     ls_it = 1;
@@ -611,18 +647,14 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const DState& loc, const W
 
 // solver.py:2155-2355 for N = 1 or 3 candidate steps
 template <int N>
-__device__ inline void wave_ls_eval_r(const DModel& m, const DState& loc, const WorkRows& R, int lane, const float* al,
-                                      LsP* out, int& ls_it) {
+__device__ inline void wave_ls_eval_r(const DModel& m, const LState& loc, const WorkRows& R, int lane, const float* al,
+                                      LsP* out, int& ls_it, float* tile) {
     const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
     const float b0 = LF(quad_gauss, 0) + LF(eq_sum, 0), b1 = LF(quad_gauss, 1) + LF(eq_sum, 1),
                 b2 = LF(quad_gauss, 2) + LF(eq_sum, 2);
-    float T[3 * N];
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-        T[3 * k] = b0;
-        T[3 * k + 1] = b1;
-        T[3 * k + 2] = b2;
-    }
+    // lane 3k + j (j < 3) sums term j of candidate k over the rows, starting from b_j
+    const int lj = lane % 3;
+    float acc = lj == 0 ? b0 : lj == 1 ? b1 : b2;
     float A[N];
 #pragma unroll
     for (int k = 0; k < N; k++) A[k] = al[k];
@@ -660,8 +692,11 @@ __device__ inline void wave_ls_eval_r(const DModel& m, const DState& loc, const 
                 }
             }
         }
-        lane_seq_sums<3 * N>(T, v, max(0, ne - c0), min(kWave, nc - c0));
+        tile_seq_sums<3 * N>(acc, v, max(0, ne - c0), min(kWave, nc - c0), tile, lane);
     }
+    float T[3 * N];
+#pragma unroll
+    for (int q = 0; q < 3 * N; q++) T[q] = readlane_f(acc, q);
 #pragma unroll
     for (int k = 0; k < N; k++) {
         const float a = A[k];
@@ -674,7 +709,7 @@ __device__ inline void wave_ls_eval_r(const DModel& m, const DState& loc, const 
 
 // solver.py:2427-2597 func_linesearch_batch (uniform control flow)
 template <int KD>
-__device__ inline float wave_linesearch(const DModel& m, const DState& loc, const WorkRows& R, int lane) {
+__device__ inline float wave_linesearch(const DModel& m, const LState& loc, const WorkRows& R, int lane, float* tile) {
     const int D = m.D;
     float sn = 0.f;
     for (int d = 0; d < D; d++) sn = sn + LF(search, d) * LF(search, d);
@@ -689,11 +724,11 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
         return 0.f;
     }
     LsP p0;
-    if constexpr (KD > 0) p0 = wave_ls_init_r<KD>(m, loc, R, lane, ls_it);
+    if constexpr (KD > 0) p0 = wave_ls_init_r<KD>(m, loc, R, lane, ls_it, tile);
     else p0 = wave_ls_init(m, loc, R, lane, ls_it);
     LsP p1;
     float a = p0.alpha - p0.grad / p0.hess;
-    if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &a, &p1, ls_it);
+    if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &a, &p1, ls_it, tile);
     else wave_ls_eval(m, loc, R, lane, &a, 1, &p1, ls_it);
     if (p0.cost < p1.cost) p1 = p0;
     bool done = false;
@@ -710,7 +745,7 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
             p2 = p1;
             p2update = 1;
             float an = p1.alpha - p1.grad / p1.hess;
-            if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &an, &p1, ls_it);
+            if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &an, &p1, ls_it, tile);
             else wave_ls_eval(m, loc, R, lane, &an, 1, &p1, ls_it);
             if (fabsf(p1.grad) < gtol) {
                 res = p1.alpha;
@@ -733,7 +768,7 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
         float al[3] = {p1.alpha - p1.grad / p1.hess, p1.alpha, (p1.alpha + p2.alpha) * 0.5f};
         while (ls_it < m.ls_iterations) {
             LsP c[3];
-            if constexpr (KD > 0) wave_ls_eval_r<3>(m, loc, R, lane, al, c, ls_it);
+            if constexpr (KD > 0) wave_ls_eval_r<3>(m, loc, R, lane, al, c, ls_it, tile);
             else wave_ls_eval(m, loc, R, lane, al, 3, c, ls_it);
             float n1 = al[0], n2 = al[1];
             bool found = false;
@@ -780,7 +815,7 @@ __device__ inline float wave_linesearch(const DModel& m, const DState& loc, cons
 
 // solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
 template <int KD>
-__device__ inline void wave_solve(const DModel& m, const DState& loc, const DState& st, int b, const WorkRows& R,
+__device__ inline void wave_solve(const DModel& m, const LState& loc, const DState& st, int b, const WorkRows& R,
                                   int lane, float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
@@ -812,7 +847,7 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
         const float tol = (m.meaninertia * (float)max(1, D)) * m.tolerance;
         for (int it = 0; it < m.iterations; it++) {
             n_iter++;
-            float alpha = wave_linesearch<KD>(m, loc, R, lane);
+            float alpha = wave_linesearch<KD>(m, loc, R, lane, tile);
             if (fabsf(alpha) < m.eps) {
                 improved = 0;
             } else {
@@ -827,7 +862,7 @@ __device__ inline void wave_solve(const DModel& m, const DState& loc, const DSta
                 if constexpr (KD > 0) {
                     cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev, tile);
                     LF(cost, 0) = cost;
-                    degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane);
+                    degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane, tile);
                 } else {
                     cost = wave_update_constraint(m, loc, R, lane, prev);
                     LF(cost, 0) = cost;

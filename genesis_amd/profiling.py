@@ -1,0 +1,154 @@
+"""Summaries of rocprofv3 output for the step kernels.
+
+* ``kernel_stats(dir)``: the per-kernel duration table of a ``rocprofv3 --kernel-trace --stats --output-format csv``
+  run (``*_kernel_stats.csv``);
+* ``counter_per_dispatch(dir, counter, match)``: mean value per dispatch of one counter of a ``rocprofv3 --pmc
+  <counter> --output-format csv`` run (``*_counter_collection.csv``), over the dispatches whose kernel name contains
+  ``match``;
+* ``traffic(...)``: HBM bytes per launch of a kernel from a FETCH_SIZE run and a WRITE_SIZE run, each scaled by what
+  the same counter reads for a streaming copy of known size in the same run (the calibration kernel);
+* ``source_digest()``: digest of the kernel sources and build flags, recorded with a profile so that a report can tell
+  whether the profile belongs to the kernels it is reporting on.
+
+Command line: ``python -m genesis_amd.profiling --trace DIR --fetch DIR --write DIR --kernel k_fused_wave
+--copy-bytes N --out profiles/NAME`` writes ``NAME.md`` (tables and figures) and ``NAME.json`` (the figures).
+"""
+
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import hashlib
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CALIBRATION_MATCH = "elementwise"  # torch's elementwise kernel that bench.py's copy calibration launches
+
+
+def source_digest() -> str:
+    from .build_lib import FLAGS
+
+    h = hashlib.sha256(" ".join(FLAGS).encode())
+    csrc = os.path.join(HERE, "csrc")
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(name.encode())
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _rows(directory: str, suffix: str) -> list[dict]:
+    files = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise FileNotFoundError(f"no *{suffix} under {directory}")
+    out = []
+    for path in files:
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                out.append({k.strip().lower(): v for k, v in r.items() if k is not None})
+    return out
+
+
+def kernel_stats(directory: str) -> list[dict]:
+    """[{name, calls, avg_us, total_ms, share}] sorted by total time."""
+    rows = _rows(directory, "kernel_stats.csv")
+    out = []
+    for r in rows:
+        out.append({"name": r["name"], "calls": int(r["calls"]), "avg_us": float(r["averagens"]) * 1e-3,
+                    "total_ms": float(r["totaldurationns"]) * 1e-6, "share": float(r.get("percentage", 0.0))})
+    out.sort(key=lambda x: -x["total_ms"])
+    return out
+
+
+def counter_per_dispatch(directory: str, counter: str, match: str, min_value: float = 0.0) -> tuple[float, int]:
+    """Mean of ``counter`` over the dispatches of kernels whose name contains ``match`` (and whose value is at least
+    ``min_value``); returns (mean, number of dispatches)."""
+    vals: dict[str, float] = {}
+    for r in _rows(directory, "counter_collection.csv"):
+        if r.get("counter_name") != counter or match not in r.get("kernel_name", ""):
+            continue
+        key = r.get("dispatch_id") or r.get("correlation_id")
+        vals[key] = vals.get(key, 0.0) + float(r["counter_value"])
+    sel = [v for v in vals.values() if v >= min_value]
+    if not sel:
+        raise ValueError(f"no dispatch of *{match}* with {counter}")
+    return sum(sel) / len(sel), len(sel)
+
+
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: int) -> dict:
+    """HBM bytes per launch of ``kernel``. FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by
+    copy_bytes / (what it reads for the calibration copy, which reads and writes copy_bytes)."""
+    out = {"kernel": kernel, "copy_bytes": copy_bytes}
+    total = 0.0
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        k_kib, n_k = counter_per_dispatch(d, name, kernel)
+        c_kib, n_c = counter_per_dispatch(d, name, CALIBRATION_MATCH, min_value=0.5 * copy_bytes / 1024)
+        factor = copy_bytes / (c_kib * 1024.0)
+        out[name] = {"kernel_kib_per_dispatch": k_kib, "kernel_dispatches": n_k, "copy_kib_per_dispatch": c_kib,
+                     "copy_dispatches": n_c, "factor": factor, "bytes_per_launch": k_kib * 1024.0 * factor}
+        total += k_kib * 1024.0 * factor
+    out["bytes_per_launch"] = total
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--kernel", default="k_fused_wave")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--bench-json", default=None, help="bench.py output of the traced run")
+    ap.add_argument("--title", default="")
+    ap.add_argument("--commands", default="", help="text of the commands that produced the runs")
+    ap.add_argument("--out", required=True, help="output path without extension")
+    a = ap.parse_args()
+    stats = kernel_stats(a.trace)
+    tr = traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
+    k = next(s for s in stats if a.kernel in s["name"])
+    bench = None
+    if a.bench_json:
+        with open(a.bench_json) as f:
+            lines = [ln for ln in f if ln.startswith("{")]
+        bench = json.loads(lines[-1]) if lines else None
+    fig = {"kernel": a.kernel, "avg_us": k["avg_us"], "calls": k["calls"], "traffic": tr,
+           "source_digest": source_digest(), "title": a.title}
+    if bench:
+        rf = bench["roofline"]
+        fig["bench"] = {"value": bench["value"], "kernel_ms": rf["kernel_ms"],
+                        "bytes_per_env_step": rf["bytes_per_env_step"], "n_envs": bench["config"]["n_envs"]}
+        fig["algorithmic_bytes_per_launch"] = rf["bytes_per_env_step"] * bench["config"]["n_envs"]
+    with open(a.out + ".json", "w") as f:
+        json.dump(fig, f, indent=1)
+    lines = [a.title or f"{a.kernel}: rocprofv3 summary", "=" * 100, "", "Commands (repo root, one MI355X):", ""]
+    lines += ["  " + ln for ln in a.commands.strip().split("\n")] + [""]
+    lines += ["Kernel trace (--kernel-trace --stats):", "",
+              f"{'kernel':60s} {'calls':>7s} {'avg us':>10s} {'total ms':>10s} {'share':>7s}"]
+    for s in stats[:12]:
+        nm = s["name"] if len(s["name"]) <= 60 else s["name"][:57] + "..."
+        lines.append(f"{nm:60s} {s['calls']:7d} {s['avg_us']:10.1f} {s['total_ms']:10.2f} {s['share']:6.1f}%")
+    lines += ["", f"HBM traffic of {a.kernel} (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB per dispatch):", ""]
+    for name in ("FETCH_SIZE", "WRITE_SIZE"):
+        t = tr[name]
+        lines.append(f"  {name}: kernel {t['kernel_kib_per_dispatch']:.1f} KiB/dispatch ({t['kernel_dispatches']} "
+                     f"dispatches); calibration copy of {a.copy_bytes} B reads {t['copy_kib_per_dispatch']:.1f} KiB "
+                     f"({t['copy_dispatches']} dispatches) -> factor {t['factor']:.4f} -> "
+                     f"{t['bytes_per_launch'] / 1e6:.3f} MB/launch")
+    lines.append(f"  traffic: {tr['bytes_per_launch'] / 1e6:.3f} MB per launch")
+    if bench:
+        alg = fig["algorithmic_bytes_per_launch"]
+        lines += ["", f"Algorithmic bytes per launch: {rf['bytes_per_env_step']:.1f} B/env-step x "
+                      f"{bench['config']['n_envs']} envs = {alg / 1e6:.3f} MB; traffic / algorithmic = "
+                      f"{tr['bytes_per_launch'] / alg:.2f}",
+                  f"bench.py on the traced run: kernel_ms {rf['kernel_ms']:.4f} (HIP events on the launch stream) vs "
+                  f"rocprof average {k['avg_us'] / 1e3:.4f} ms"]
+    lines += ["", f"source digest (kernel sources + build flags): {fig['source_digest']}"]
+    with open(a.out + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()

@@ -190,7 +190,14 @@ class Scene:
         elif self.n_envs == 0 and v.ndim == 1:
             v = v[None]
         elif v.ndim == 1:
-            v = v[None].expand(n_env, len(rows))
+            v = v[None]
+        if v.ndim > 2:
+            raise GenesisException(f"Invalid input shape: {tuple(v.shape)}. Expecting at most 2D tensor.")
+        try:  # utils/misc.py:816-830: the value must broadcast to (len(envs_idx), len(dofs_idx))
+            v = v.expand(n_env, len(rows))
+        except RuntimeError:
+            raise GenesisException(f"Invalid input shape: {tuple(v.shape)}. Dimension 0 consistent with "
+                                   f"len(envs_idx)={n_env}, dimension 1 with len(dofs_idx)={len(rows)}") from None
         if idx is None:
             f[rows] = v.t().to(f.dtype)
         else:
