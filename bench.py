@@ -246,7 +246,9 @@ def main():
     print(f"[bench] fused: {wall_f:.3f}s for {steps} steps", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain:
-        wall_p, gpu_ms_p, lat_p, nb_p, _ = run_path(wl, dm, B, steps, args.warmup, rt.RS_MODE_PLAIN, targets,
+        # the per-stage path on its own layout ([row][env] planes, coalesced for one lane per environment)
+        dm_plain = rt.DeviceModel(model, rt.LAYOUT_PLANES)
+        wall_p, gpu_ms_p, lat_p, nb_p, _ = run_path(wl, dm_plain, B, steps, args.warmup, rt.RS_MODE_PLAIN, targets,
                                                     use_graph=False, n_lat=50)
         plain = {"value": B * steps / wall_p, "ms_per_step": 1e3 * wall_p / steps,
                  "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))}

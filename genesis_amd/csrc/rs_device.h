@@ -30,7 +30,8 @@ struct DModel {
 
 // ------------------------------------------------------------------------------------------------- state view
 struct DState {
-    int B;  // number of environments (plane stride)
+    int B;       // number of environments (plane stride)
+    int64_t LS;  // > 0: per-environment (local-kind) fields are env-major records of LS elements; 0: [row][env] planes
 #define RS_DS_PTR(name, is_int, kind, width) typename std::conditional<is_int, int32_t, float>::type* name;
     RS_STATE_FIELDS(RS_DS_PTR)
 #undef RS_DS_PTR
@@ -79,8 +80,31 @@ __device__ __forceinline__ typename FieldOf<F>::T* fptr(const LState& s) {
     static_assert(is_local_kind(FieldOf<F>::count_kind), "a local view reaches private-region fields only");
     return (typename FieldOf<F>::T*)(s.base + s.off[F]);
 }
-// pointer to field `field` of state s (a DState: the [row][env] plane; an LState: the private-region copy)
+// pointer to field `field` of state s (a DState: the field's first element; an LState: the private-region copy)
 #define FP(s, field) fptr<FID_##field>(s)
+
+// Environment b's elements of a field: element r at p[r * rs]. In a DState a local-kind field is either a plane
+// ([row][env], rs = B) or, with env-major records (LS > 0), a slice of environment b's record (rs = 1); every other
+// field is a plane. In an LState every field is a private-region slice.
+template <class T>
+struct FView {
+    T* p;
+    int rs;
+    __device__ __forceinline__ T& operator[](int64_t r) const { return p[r * rs]; }
+};
+template <int F>
+__device__ __forceinline__ FView<typename FieldOf<F>::T> field_view(const DState& s, int b) {
+    typename FieldOf<F>::T* p = fptr<F>(s);
+    if constexpr (is_local_kind(FieldOf<F>::count_kind)) {
+        if (s.LS > 0) return {p + (size_t)b * s.LS, 1};
+    }
+    return {p + b, s.B};
+}
+template <int F>
+__device__ __forceinline__ FView<typename FieldOf<F>::T> field_view(const LState& s, int) {
+    return {fptr<F>(s), 1};
+}
+#define FV(s, field, b) field_view<FID_##field>(s, b)
 
 struct f3 {
     float x, y, z;
@@ -285,7 +309,7 @@ __device__ __forceinline__ m33 mmat3(cfptr a, int i) {
 
 // --------------------------------------------------------------------------------- state plane accessors
 // element (row) of a field for environment b: field[row * B + b]
-#define SF(field, row) FP(st, field)[(size_t)(row) * st.B + b]
+#define SF(field, row) FV(st, field, b)[(row)]
 
 __device__ __forceinline__ f3 ld3(const float* f, int i, int B, int b) {
     return {f[(size_t)(3 * i) * B + b], f[(size_t)(3 * i + 1) * B + b], f[(size_t)(3 * i + 2) * B + b]};
@@ -315,12 +339,45 @@ __device__ __forceinline__ void st9(float* f, int i, int B, int b, const m33& m)
 #pragma unroll
     for (int k = 0; k < 9; k++) f[(size_t)(9 * i + k) * B + b] = m.a[k / 3][k % 3];
 }
-#define LD3(field, i) ld3(FP(st, field), (i), st.B, b)
-#define ST3(field, i, v) st3(FP(st, field), (i), st.B, b, (v))
-#define LD4(field, i) ld4(FP(st, field), (i), st.B, b)
-#define ST4(field, i, v) st4(FP(st, field), (i), st.B, b, (v))
-#define LD9(field, i) ld9(FP(st, field), (i), st.B, b)
-#define ST9(field, i, v) st9(FP(st, field), (i), st.B, b, (v))
+template <class T>
+__device__ __forceinline__ f3 ld3(FView<T> f, int i) {
+    return {f[3 * i], f[3 * i + 1], f[3 * i + 2]};
+}
+template <class T>
+__device__ __forceinline__ void st3(FView<T> f, int i, f3 v) {
+    f[3 * i] = v.x;
+    f[3 * i + 1] = v.y;
+    f[3 * i + 2] = v.z;
+}
+template <class T>
+__device__ __forceinline__ f4q ld4(FView<T> f, int i) {
+    return {f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]};
+}
+template <class T>
+__device__ __forceinline__ void st4(FView<T> f, int i, f4q q) {
+    f[4 * i] = q.w;
+    f[4 * i + 1] = q.x;
+    f[4 * i + 2] = q.y;
+    f[4 * i + 3] = q.z;
+}
+template <class T>
+__device__ __forceinline__ m33 ld9(FView<T> f, int i) {
+    m33 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.a[k / 3][k % 3] = f[9 * i + k];
+    return r;
+}
+template <class T>
+__device__ __forceinline__ void st9(FView<T> f, int i, const m33& m) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[9 * i + k] = m.a[k / 3][k % 3];
+}
+#define LD3(field, i) ld3(FV(st, field, b), (i))
+#define ST3(field, i, v) st3(FV(st, field, b), (i), (v))
+#define LD4(field, i) ld4(FV(st, field, b), (i))
+#define ST4(field, i, v) st4(FV(st, field, b), (i), (v))
+#define LD9(field, i) ld9(FV(st, field, b), (i))
+#define ST9(field, i, v) st9(FV(st, field, b), (i), (v))
 
 constexpr int CTRL_POSITION = 0, CTRL_VELOCITY = 1, CTRL_FORCE = 2;
 constexpr int JT_FIXED = 0, JT_REVOLUTE = 1, JT_PRISMATIC = 2, JT_SPHERICAL = 3, JT_FREE = 4;

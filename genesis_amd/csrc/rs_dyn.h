@@ -295,22 +295,21 @@ __device__ inline void dev_factor_mass(const DModel& m, const S& st, int b) {
 
 // forward_dynamics.py:813-894 func_solve_mass_entity: out = M^-1 vec (vec, out: dof planes)
 template <class S>
-__device__ inline void dev_solve_mass(const DModel& m, const S& st, int b, const float* vec, float* out) {
+__device__ inline void dev_solve_mass(const DModel& m, const S& st, int b, FView<float> vec, FView<float> out) {
     const int D = m.D;
-    const int B = st.B;
     for (int e = 0; e < m.E; e++) {
         int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
         for (int ii = 0; ii < d1 - d0; ii++) {
             int i = d1 - ii - 1;
-            float cur = vec[(size_t)i * B + b];
-            for (int j = i + 1; j < d1; j++) cur = cur - SF(mass_L, j * D + i) * out[(size_t)j * B + b];
-            out[(size_t)i * B + b] = cur;
+            float cur = vec[i];
+            for (int j = i + 1; j < d1; j++) cur = cur - SF(mass_L, j * D + i) * out[j];
+            out[i] = cur;
         }
-        for (int i = d0; i < d1; i++) out[(size_t)i * B + b] = out[(size_t)i * B + b] * SF(mass_D_inv, i);
+        for (int i = d0; i < d1; i++) out[i] = out[i] * SF(mass_D_inv, i);
         for (int i = d0; i < d1; i++) {
-            float cur = out[(size_t)i * B + b];
-            for (int j = d0; j < i; j++) cur = cur - SF(mass_L, i * D + j) * out[(size_t)j * B + b];
-            out[(size_t)i * B + b] = cur;
+            float cur = out[i];
+            for (int j = d0; j < i; j++) cur = cur - SF(mass_L, i * D + j) * out[j];
+            out[i] = cur;
         }
     }
 }
@@ -451,7 +450,7 @@ __device__ inline void dev_rne_and_qacc(const DModel& m, const S& st, int b) {
             SF(qf_smooth, d) = f;
         }
     }
-    dev_solve_mass(m, st, b, FP(st, force), FP(st, acc_smooth));
+    dev_solve_mass(m, st, b, FV(st, force, b), FV(st, acc_smooth, b));
     for (int d = 0; d < m.D; d++) SF(acc, d) = SF(acc_smooth, d);
 }
 

@@ -59,7 +59,8 @@ struct rs_model {
     FusedArgs fa;       // fused-kernel layout and launch shape
     int use_lds = 0;    // private regions in LDS
     int schedule = 1;   // cost-ordered environment queues
-    int occupancy = 0;  // 0: compiler's register budget; 4: at least four wavefronts per SIMD
+    int occupancy = 0;  // 0: compiler's register budget; 3 or 4: at least that many wavefronts per SIMD
+    int layout = RS_LAYOUT_ENV_RECORDS;  // state layout of the per-environment (local-kind) fields
     int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
 };
 
@@ -153,15 +154,38 @@ static int64_t work_floats(const rs_model_header& h) {
 }
 
 // ------------------------------------------------------------------------------------------------ state views
-// Local-kind fields (first in the directory) are packed back to back; every other plane starts 256-byte aligned.
-static void layout(const rs_model_header& h, int n_envs, int64_t* offs, int64_t* total) {
-    int64_t off = 0;
+// Local-kind fields come first in the directory. RS_LAYOUT_PLANES: they are [rows][n_envs] planes packed back to
+// back. RS_LAYOUT_ENV_RECORDS: each environment's local rows form one record of record_floats() elements (the rows
+// of the local fields back to back, padded to 128 bytes) and the records follow each other; offsets[i] of a local
+// field is then its byte offset inside a record. Every other field is a [rows][n_envs] plane starting 256-byte
+// aligned after them.
+static int64_t local_rows(const rs_model_header& h) {
+    int64_t n = 0;
+    for (int i = 0; i < k_n_state_fields; i++)
+        if (is_local_kind(k_state_fields[i].count)) n += field_rows(h, i);
+    return n;
+}
+static int64_t record_floats(const rs_model* m) {
+    return m->layout == RS_LAYOUT_ENV_RECORDS ? (local_rows(m->h) + 31) / 32 * 32 : 0;
+}
+
+static void layout(const rs_model* m, int n_envs, int64_t* offs, int64_t* total) {
+    const int64_t rec = record_floats(m);
+    int64_t off = 0, in_rec = 0;
     for (int i = 0; i < k_n_state_fields; i++) {
-        const int64_t bytes = field_rows(h, i) * n_envs * 4;
+        const int64_t rows = field_rows(m->h, i);
         const bool local = is_local_kind(k_state_fields[i].count);
-        if (!local) off = (off + 255) / 256 * 256;
+        if (local && rec > 0) {
+            offs[i] = in_rec * 4;
+            in_rec += rows;
+            continue;
+        }
+        if (!local) {
+            if (off == 0 && rec > 0) off = rec * n_envs * 4;
+            off = (off + 255) / 256 * 256;
+        }
         offs[i] = off;
-        off += bytes;
+        off += rows * n_envs * 4;
     }
     *total = (off + 255) / 256 * 256;
 }
@@ -169,9 +193,10 @@ static void layout(const rs_model_header& h, int n_envs, int64_t* offs, int64_t*
 static DState make_state(const rs_model* m, void* base, int n_envs) {
     std::vector<int64_t> offs(k_n_state_fields);
     int64_t total;
-    layout(m->h, n_envs, offs.data(), &total);
+    layout(m, n_envs, offs.data(), &total);
     DState s;
     s.B = n_envs;
+    s.LS = record_floats(m);
     int i = 0;
     char* p = (char*)base;
 #define RS_X(name, is_int, kind, width) s.name = (decltype(s.name))(p + offs[i++]);
@@ -254,7 +279,7 @@ __global__ void __launch_bounds__(256) k_aabb(DModel m, DState st) {
 
 __global__ void k_errno_or(DState st, int32_t* out) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = b < st.B ? st.errbits[b] : 0;
+    int v = b < st.B ? FV(st, errbits, b)[0] : 0;
     if (v) atomicOr(out, v);
 }
 
@@ -295,14 +320,15 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     if (fa.sched) hipLaunchKernelGGL(k_sched_flip, dim3(1), dim3(1), 0, s, st);
     const dim3 grid((unsigned)((st.B + epb - 1) / epb)), block((unsigned)(kWave * epb));
     const size_t lds = (size_t)epb * fa.loc_floats * 4;
-    if (m->use_lds && m->occupancy == 4)
-        hipLaunchKernelGGL((k_fused_wave<true, 4>), grid, block, lds, s, m->dm, st, fa);
-    else if (m->use_lds)
-        hipLaunchKernelGGL((k_fused_wave<true, 0>), grid, block, lds, s, m->dm, st, fa);
-    else if (m->occupancy == 4)
-        hipLaunchKernelGGL((k_fused_wave<false, 4>), grid, block, 0, s, m->dm, st, fa);
-    else
-        hipLaunchKernelGGL((k_fused_wave<false, 0>), grid, block, 0, s, m->dm, st, fa);
+    if (m->use_lds) {
+        if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<true, 4>), grid, block, lds, s, m->dm, st, fa);
+        else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3>), grid, block, lds, s, m->dm, st, fa);
+        else hipLaunchKernelGGL((k_fused_wave<true, 0>), grid, block, lds, s, m->dm, st, fa);
+    } else {
+        if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<false, 4>), grid, block, 0, s, m->dm, st, fa);
+        else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3>), grid, block, 0, s, m->dm, st, fa);
+        else hipLaunchKernelGGL((k_fused_wave<false, 0>), grid, block, 0, s, m->dm, st, fa);
+    }
     HIP_OK(hipGetLastError());
     return RS_OK;
 }
@@ -434,7 +460,7 @@ int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     m->fa.epb = c->envs_per_block;
     m->use_lds = c->use_lds != 0;
     m->schedule = c->schedule != 0;
-    m->occupancy = c->occupancy == 4 ? 4 : 0;
+    m->occupancy = (c->occupancy == 3 || c->occupancy == 4) ? c->occupancy : 0;
     return RS_OK;
 }
 
@@ -453,8 +479,27 @@ void rs_model_destroy(rs_model* m) {
 int rs_state_layout(const rs_model* m, int n_envs, int64_t* offsets, int64_t* total_bytes) {
     if (!m || n_envs < 1 || !total_bytes) return fail(RS_E_INVALID, "rs_state_layout: bad argument");
     std::vector<int64_t> offs(k_n_state_fields);
-    layout(m->h, n_envs, offs.data(), total_bytes);
+    layout(m, n_envs, offs.data(), total_bytes);
     if (offsets) std::memcpy(offsets, offs.data(), sizeof(int64_t) * k_n_state_fields);
+    return RS_OK;
+}
+
+int rs_state_record_bytes(const rs_model* m, int64_t* bytes) {
+    if (!m || !bytes) return fail(RS_E_INVALID, "rs_state_record_bytes: null argument");
+    *bytes = record_floats(m) * 4;
+    return RS_OK;
+}
+
+int rs_model_set_state_layout(rs_model* m, int layout) {
+    if (!m || (layout != RS_LAYOUT_PLANES && layout != RS_LAYOUT_ENV_RECORDS))
+        return fail(RS_E_INVALID, "rs_model_set_state_layout: bad argument");
+    m->layout = layout;
+    return RS_OK;
+}
+
+int rs_model_get_state_layout(const rs_model* m, int* layout) {
+    if (!m || !layout) return fail(RS_E_INVALID, "rs_model_get_state_layout: null argument");
+    *layout = m->layout;
     return RS_OK;
 }
 
@@ -463,7 +508,7 @@ int rs_state_init(const rs_model* m, void* state, int n_envs, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     int64_t total;
     std::vector<int64_t> offs(k_n_state_fields);
-    layout(m->h, n_envs, offs.data(), &total);
+    layout(m, n_envs, offs.data(), &total);
     HIP_OK(hipMemsetAsync(state, 0, total, s));
     DState st = make_state(m, state, n_envs);
     hipLaunchKernelGGL(k_init, env_grid(n_envs), dim3(kBlock), 0, s, m->dm, st);

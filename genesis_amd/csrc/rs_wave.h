@@ -130,11 +130,37 @@ __device__ __forceinline__ LState make_local(float* base, const FusedArgs& fa) {
     return LState{base, (const RS_CONST int*)fa.loc_off};
 }
 
-// Copy environment b's local rows between the packed local planes (which start at the slab base, the first local
-// field) and its private region: local row k is slab element k * n_envs + b. Rows are copied as raw 32-bit words.
+// Copy environment b's local rows between the slab and its private region, as raw 32-bit words. With env-major
+// records (st.LS > 0) the rows are one contiguous, 128-byte aligned record (copied 16 bytes per lane and
+// instruction); with planes, local row k is slab element k * n_envs + b (the packed local planes start at the slab
+// base, the first local field).
 template <bool kIn, int kUnroll = 8>
 __device__ __forceinline__ void stage_local(const DState& st, uint32_t* priv, int n_local, int b, int lane) {
     uint32_t* slab = (uint32_t*)(void*)st.qpos;
+    if (st.LS > 0) {
+        uint32_t* rec = slab + (size_t)b * st.LS;
+        const int n4 = n_local >> 2;
+        int k = lane;
+        for (; k + (kUnroll - 1) * kWave < n4; k += kUnroll * kWave) {
+            uint4 v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) v[u] = kIn ? ((const uint4*)rec)[k + u * kWave] : ((const uint4*)priv)[k + u * kWave];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (kIn) ((uint4*)priv)[k + u * kWave] = v[u];
+                else ((uint4*)rec)[k + u * kWave] = v[u];
+            }
+        }
+        for (; k < n4; k += kWave) {
+            if (kIn) ((uint4*)priv)[k] = ((const uint4*)rec)[k];
+            else ((uint4*)rec)[k] = ((const uint4*)priv)[k];
+        }
+        for (int t = 4 * n4 + lane; t < n_local; t += kWave) {
+            if (kIn) priv[t] = rec[t];
+            else rec[t] = priv[t];
+        }
+        return;
+    }
     const size_t B = (size_t)st.B;
     int k = lane;
     for (; k + (kUnroll - 1) * kWave < n_local; k += kUnroll * kWave) {

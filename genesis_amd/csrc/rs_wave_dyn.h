@@ -222,7 +222,7 @@ __device__ inline void wave_rne_and_qacc(const DModel& m, const LState& loc, int
         LF(qf_smooth, d) = f;
     }
     wave_sync();
-    dev_solve_mass(m, loc, 0, FP(loc, force), FP(loc, acc_smooth));
+    dev_solve_mass(m, loc, 0, FV(loc, force, 0), FV(loc, acc_smooth, 0));
     for (int d = 0; d < m.D; d++) LF(acc, d) = LF(acc_smooth, d);
 }
 
@@ -370,9 +370,104 @@ __device__ inline void wave_update_geoms(const DModel& m, const LState& loc, int
     }
 }
 
-// forward kinematics (the chain walk runs on every lane), then COM, geoms and velocities split across lanes
+// forward_kinematics.py:580-736 with the per-joint work that does not depend on the chain done first, one lane per
+// joint: the joint's local rotation (revolute: from its angle; spherical and free: from its coordinates) and its
+// dof positions. The chain walk then runs on every lane with the same arithmetic as dev_forward_kinematics, taking a
+// joint's precomputed values from its lane and keeping the anchor, axis and the previous link's pose in registers
+// rather than reading back what it has just stored. Models with more than 64 joints take dev_forward_kinematics.
+__device__ inline void wave_forward_kinematics(const DModel& m, const LState& loc, int lane) {
+    if (m.J > kWave) {
+        dev_forward_kinematics(m, loc, 0);
+        return;
+    }
+    const LState& st = loc;
+    const int b = 0;
+    f4q jq{1.f, 0.f, 0.f, 0.f};  // lane j: local rotation of joint j (free: the normalised orientation)
+    float jd = 0.f;               // lane j: revolute / prismatic displacement
+    if (lane < m.J) {
+        const int j = lane;
+        const int jt = m.joint_type[j], qs = m.joint_q_start[j], ds = m.joint_dof_start[j];
+        if (jt == JT_FREE) {
+            f4q q{SF(qpos, qs + 3), SF(qpos, qs + 4), SF(qpos, qs + 5), SF(qpos, qs + 6)};
+            float n = sqrtf(nsq4(q));
+            jq = {q.w / n, q.x / n, q.y / n, q.z / n};
+            f3 xyz = quat_xyz(jq, m.eps);
+            for (int k = 0; k < 3; k++) {
+                SF(dof_pos, ds + k) = SF(qpos, qs + k);
+                SF(dof_pos, ds + 3 + k) = xyz[k];
+            }
+        } else if (jt == JT_SPHERICAL) {
+            jq = {SF(qpos, qs), SF(qpos, qs + 1), SF(qpos, qs + 2), SF(qpos, qs + 3)};
+            f3 xyz = quat_xyz(jq, m.eps);
+            for (int k = 0; k < 3; k++) SF(dof_pos, ds + k) = xyz[k];
+        } else if (jt == JT_REVOLUTE) {
+            jd = SF(qpos, qs) - m.qpos0[qs];
+            SF(dof_pos, ds) = jd;
+            jq = rotvec_quat(mvec3(m.dof_motion_ang, ds) * jd, m.eps);
+        } else if (jt == JT_PRISMATIC) {
+            jd = SF(qpos, qs) - m.qpos0[qs];
+            SF(dof_pos, ds) = jd;
+        }
+    }
+    f3 prev_pos = zero3();
+    f4q prev_quat{1.f, 0.f, 0.f, 0.f};
+    int prev = -1;  // link whose pose prev_pos / prev_quat hold (stored in this walk)
+    for (int l = 0; l < m.L; l++) {
+        f3 pos = mvec3(m.link_pos, l);
+        f4q quat = mquat(m.link_quat, l);
+        const int par = m.link_parent[l];
+        if (par != -1) {
+            const f3 pp = par == prev ? prev_pos : LD3(links_pos, par);
+            const f4q pq = par == prev ? prev_quat : LD4(links_quat, par);
+            pos = pp + qrot(mvec3(m.link_pos, l), pq);
+            quat = qtrans(mquat(m.link_quat, l), pq);
+        }
+        for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
+            const int jt = m.joint_type[j], qs = m.joint_q_start[j], ds = m.joint_dof_start[j];
+            const f4q ql{readlane_f(jq.w, j), readlane_f(jq.x, j), readlane_f(jq.y, j), readlane_f(jq.z, j)};
+            const float dp = readlane_f(jd, j);
+            f3 xanchor = zero3(), xaxis = zero3();
+            if (jt == JT_FREE) {
+                xanchor = mk3(SF(qpos, qs), SF(qpos, qs + 1), SF(qpos, qs + 2));
+                xaxis = mk3(0.f, 0.f, 1.f);
+            } else if (jt != JT_FIXED) {
+                f3 axis = mk3(0.f, 0.f, 1.f);
+                if (jt == JT_REVOLUTE) axis = mvec3(m.dof_motion_ang, ds);
+                else if (jt == JT_PRISMATIC) axis = mvec3(m.dof_motion_vel, ds);
+                xanchor = qrot(mvec3(m.joint_pos, j), quat) + pos;
+                xaxis = qrot(axis, quat);
+            }
+            if (jt != JT_FIXED) {
+                ST3(xanchor, j, xanchor);
+                ST3(xaxis, j, xaxis);
+            }
+            if (jt == JT_FREE) {
+                pos = xanchor;
+                quat = ql;
+            } else if (jt == JT_SPHERICAL) {
+                quat = qtrans(ql, quat);
+                pos = xanchor - qrot(mvec3(m.joint_pos, j), quat);
+            } else if (jt == JT_REVOLUTE) {
+                quat = qtrans(ql, quat);
+                pos = xanchor - qrot(mvec3(m.joint_pos, j), quat);
+            } else if (jt == JT_PRISMATIC) {
+                pos = pos + xaxis * dp;
+            }
+        }
+        if (!(par == -1 && m.link_is_fixed[l])) {
+            ST3(links_pos, l, pos);
+            ST4(links_quat, l, quat);
+            prev = l;
+            prev_pos = pos;
+            prev_quat = quat;
+        }
+    }
+    wave_sync();
+}
+
+// forward kinematics, then COM, geoms and velocities split across lanes
 __device__ inline void wave_kinematics(const DModel& m, const LState& loc, int lane) {
-    dev_forward_kinematics(m, loc, 0);
+    wave_forward_kinematics(m, loc, lane);
     wave_sync();
     wave_com_links(m, loc, lane);
     wave_update_geoms(m, loc, lane);

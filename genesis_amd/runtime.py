@@ -34,6 +34,11 @@ COUNT_KINDS = [
     "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK", "SCHED",
 ]
 ENV_MAJOR_KINDS = {"WORK"}
+LOCAL_KINDS = {"ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "DOFS2", "SORT"}
+
+# state layouts of the per-environment fields (include/rigidstep.h)
+LAYOUT_PLANES = 0
+LAYOUT_ENV_RECORDS = 1
 
 
 class RigidStepError(RuntimeError):
@@ -66,6 +71,9 @@ _SIGNATURES = {
     "rs_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_int64)]),
     "rs_state_field_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "rs_state_record_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "rs_model_set_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "rs_model_get_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "rs_fused_config_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FusedConfig)]),
     "rs_fused_config_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FusedConfig)]),
     "rs_state_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
@@ -141,7 +149,9 @@ def _stream_handle(stream=None):
 class DeviceModel:
     """A compiled model uploaded to the current device (``rs_model``)."""
 
-    def __init__(self, model: CompiledModel):
+    def __init__(self, model: CompiledModel, layout: int = LAYOUT_ENV_RECORDS):
+        """``layout``: state layout of the per-environment fields of this model's slabs (LAYOUT_ENV_RECORDS, the
+        default, one record per environment; LAYOUT_PLANES, [rows][n_envs] planes)."""
         lib = load_library()
         self.model = model
         self.header, blob = pack_model(model, np.float32)
@@ -149,6 +159,8 @@ class DeviceModel:
         h = ctypes.c_void_p()
         _check(lib.rs_model_create(ctypes.byref(self.header), buf, len(blob), ctypes.byref(h)))
         self.handle = h
+        _check(lib.rs_model_set_state_layout(h, int(layout)))
+        self.layout = int(layout)
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
@@ -186,10 +198,19 @@ class StateSlab:
         self.nbytes = total.value
         device = device or torch.device("cuda", torch.cuda.current_device())
         self.buffer = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        rec = ctypes.c_int64()
+        _check(lib.rs_state_record_bytes(dmodel.handle, ctypes.byref(rec)))
+        self.record_bytes = rec.value
         self.fields = {}
         for i, ((name, is_int, kind, width), off) in enumerate(zip(self.directory, offs)):
             rows = dmodel.field_rows(i)
-            raw = self.buffer[off : off + rows * n_envs * 4].view(torch.int32 if is_int else torch.float32)
+            dtype = torch.int32 if is_int else torch.float32
+            if kind in LOCAL_KINDS and self.record_bytes > 0:
+                # env-major records: a [rows, n_envs] view with strides (1, record)
+                recs = self.buffer[: n_envs * self.record_bytes].view(dtype).view(n_envs, self.record_bytes // 4)
+                self.fields[name] = recs[:, off // 4 : off // 4 + rows].t()
+                continue
+            raw = self.buffer[off : off + rows * n_envs * 4].view(dtype)
             # [rows, n_envs] planes; the fused kernel's work field is env-major
             self.fields[name] = raw.view(n_envs, rows) if kind in ENV_MAJOR_KINDS else raw.view(rows, n_envs)
         self.init()
@@ -274,7 +295,7 @@ def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=Non
     unchanged. The winner is stored in the model and returned with the timings (ms per substep)."""
     dm = slab.dmodel
     if candidates is None:
-        candidates = [(e, lds, sch, occ) for occ in (0, 4) for sch in (True, False) for lds in (True, False)
+        candidates = [(e, lds, sch, occ) for occ in (0, 3, 4) for sch in (True, False) for lds in (True, False)
                       for e in (1, 2, 4, 8)]
     saved = dm.fused_config
     scratch = StateSlab.__new__(StateSlab)
@@ -303,7 +324,7 @@ def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=Non
     dm.set_fused_config(*best)
     def name(c):
         return "epb{}_{}{}{}".format(c[0], "lds" if c[1] else "hbm", "" if len(c) < 3 or c[2] else "_nosched",
-                                     "_occ4" if len(c) > 3 and c[3] == 4 else "")
+                                     f"_occ{c[3]}" if len(c) > 3 and c[3] else "")
 
     return {"envs_per_block": best[0], "use_lds": best[1], "schedule": best[2] if len(best) > 2 else True,
             "occupancy": best[3] if len(best) > 3 else 0, "ms_per_substep": timings[best],

@@ -12,7 +12,8 @@
  *
  * Memory: the caller owns all device memory. The model blob is copied to the device by rs_model_create and owned by
  * the returned handle. The state slab is a caller-allocated device buffer of rs_state_layout().total_bytes bytes whose
- * fields are [rows][n_envs] planes (environment index fastest), float32 or int32. All work is enqueued on the stream
+ * fields are [rows][n_envs] planes (environment index fastest), float32 or int32, except the per-environment fields,
+ * which are per-environment records by default (RS_LAYOUT_ENV_RECORDS below). All work is enqueued on the stream
  * passed by the caller; no entry point synchronises except rs_errno_reduce and the *_sync helpers.
  *
  * Return codes: 0 on success, negative on error (RS_E_*). Error text: rs_last_error().
@@ -128,9 +129,22 @@ int rs_state_field_info(int index, rs_array_info* out);
 /* Rows per environment of state field `index` for this model (count of its kind x width, at least 1). */
 int rs_state_field_rows(const rs_model* model, int index, int64_t* rows);
 
+/* State layouts of the per-environment (local-kind: one, links, joints, dofs, qs, geoms, entities, dofs^2, SAP
+ * endpoint) fields. RS_LAYOUT_PLANES: [rows][n_envs] planes like every other field (Genesis's [n, B] field layout).
+ * RS_LAYOUT_ENV_RECORDS (the default): one contiguous record per environment holding all its local rows, so that the
+ * fused kernel stages an environment with contiguous copies; element (row r, env b) of local field i is at byte
+ * offsets[i] + b * record_bytes + 4 r. Set it before creating the model's slabs: a slab is laid out, and read by
+ * every entry point, according to the model's current setting. */
+enum { RS_LAYOUT_PLANES = 0, RS_LAYOUT_ENV_RECORDS = 1 };
+int rs_model_set_state_layout(rs_model* model, int layout);
+int rs_model_get_state_layout(const rs_model* model, int* layout);
+
 /* Byte offset of every state field for n_envs environments, and the slab size. offsets has rs_state_field_count()
- * entries. */
+ * entries; for local fields in RS_LAYOUT_ENV_RECORDS the offset is inside the record of environment 0. */
 int rs_state_layout(const rs_model* model, int n_envs, int64_t* offsets, int64_t* total_bytes);
+
+/* Bytes per environment record of the local fields (0 in RS_LAYOUT_PLANES). */
+int rs_state_record_bytes(const rs_model* model, int64_t* record_bytes);
 
 /* Initialise a slab: qpos = qpos0, zero velocities, FORCE control mode, cleared caches, then forward kinematics
  * (Scene.build + reset, scene.py:841-905). */
@@ -157,14 +171,14 @@ int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_
  * envs_per_block wavefronts per workgroup (1..8), whether the environment's small fields are staged in LDS
  * (use_lds = 1, only when they fit) or in the env-major work field, and whether environments are started in order of
  * their previous substep's cost (schedule = 1; every environment's result is the same in any order), and the kernel's
- * register budget (occupancy 0 or 4). rs_fused_config_set rejects configurations the
+ * register budget (occupancy 0, 3 or 4). rs_fused_config_set rejects configurations the
  * device cannot run. The model keeps the configuration for every later RS_MODE_FUSED step. */
 typedef struct rs_fused_config {
     int32_t envs_per_block;
     int32_t use_lds;
     int32_t lds_bytes_per_env;  /* out: private-region size of one environment */
     int32_t schedule;           /* 1: start the environments that were costliest in the previous substep first */
-    int32_t occupancy;          /* 0: compiler's register budget; 4: kernel built for four wavefronts per SIMD */
+    int32_t occupancy;          /* 0: compiler's register budget; 3 or 4: kernel built for that many wavefronts per SIMD */
 } rs_fused_config;
 int rs_fused_config_get(const rs_model* model, rs_fused_config* out);
 int rs_fused_config_set(rs_model* model, const rs_fused_config* config);

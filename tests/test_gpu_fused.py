@@ -28,10 +28,10 @@ def _rt():
     return runtime
 
 
-def _franka(B, warm=20, seed=0):
+def _franka(B, warm=20, seed=0, layout=None):
     rt = _rt()
     model = franka_model()
-    dm = rt.DeviceModel(model)
+    dm = rt.DeviceModel(model, rt.LAYOUT_ENV_RECORDS if layout is None else layout)
     slab = rt.StateSlab(dm, B)
     slab["qpos"].copy_(torch.as_tensor(franka_initial_qpos(model, B, seed).T, dtype=torch.float32))
     slab["ctrl_mode"].fill_(0)
@@ -64,12 +64,13 @@ def _max_abs(a, b):
     return float(np.max(np.abs(a - b))) if a.size else 0.0
 
 
+@pytest.mark.parametrize("layout", ["records", "planes"])
 @pytest.mark.parametrize("warm", [0, 20, 60])
-def test_fused_step_equals_plain_step(warm):
+def test_fused_step_equals_plain_step(warm, layout):
     rt = _rt()
-    model, dm, a = _franka(256, warm)
+    model, dm, a = _franka(256, warm, layout=rt.LAYOUT_PLANES if layout == "planes" else rt.LAYOUT_ENV_RECORDS)
     b = _twin(rt, dm, a)
-    print(f"[fused] franka B=256 warm={warm} config={dm.fused_config}", flush=True)
+    print(f"[fused] franka B=256 warm={warm} layout={layout} config={dm.fused_config}", flush=True)
     a.step(1, rt.RS_MODE_PLAIN)
     b.step(1, rt.RS_MODE_FUSED)
     torch.cuda.synchronize()
@@ -80,13 +81,27 @@ def test_fused_step_equals_plain_step(warm):
         assert _max_abs(x, y) == 0.0
 
 
+def test_state_layouts_agree():
+    """The same 5 substeps on a slab with per-environment records and on one with planes: identical fields."""
+    rt = _rt()
+    _, _, a = _franka(96, 15, layout=rt.LAYOUT_ENV_RECORDS)
+    _, _, b = _franka(96, 15, layout=rt.LAYOUT_PLANES)
+    print("[fused] layouts: records (fused) vs planes (plain)", flush=True)
+    a.step(5, rt.RS_MODE_FUSED)
+    b.step(5, rt.RS_MODE_PLAIN)
+    torch.cuda.synchronize()
+    for name in OUTPUTS:
+        d = _max_abs(get_dev(a, name), get_dev(b, name))
+        assert d == 0.0, f"{name}: records vs planes differ by {d:.3e}"
+
+
 def test_fused_launch_shapes_agree():
     rt = _rt()
     model, dm, ref = _franka(128, 30)
     want = _twin(rt, dm, ref)
     want.step(5, rt.RS_MODE_PLAIN)
     torch.cuda.synchronize()
-    shapes = [(epb, lds, sched, occ) for occ in (0, 4) for sched in (True, False) for lds in (True, False)
+    shapes = [(epb, lds, sched, occ) for occ in (0, 3, 4) for sched in (True, False) for lds in (True, False)
               for epb in (1, 2, 4, 8)]
     for epb, lds, sched, occ in shapes:
         try:

@@ -26,10 +26,10 @@ def _rt():
     return runtime
 
 
-def _franka_slab(B, steps=0, seed=0):
+def _franka_slab(B, steps=0, seed=0, layout=None):
     rt = _rt()
     model = franka_model()
-    dm = rt.DeviceModel(model)
+    dm = rt.DeviceModel(model, rt.LAYOUT_ENV_RECORDS if layout is None else layout)
     slab = rt.StateSlab(dm, B)
     slab["qpos"].copy_(torch.as_tensor(franka_initial_qpos(model, B, seed).T, dtype=torch.float32))
     slab["ctrl_mode"].fill_(0)
@@ -61,11 +61,12 @@ def _masked(slab, oracle, name, count_name, width):
     return dev[mask], ref[mask]
 
 
-@pytest.mark.parametrize("warm_steps", [0, 20])
-def test_franka_stage_parity(warm_steps):
+@pytest.mark.parametrize("warm_steps,layout", [(0, "planes"), (20, "planes"), (20, "records")])
+def test_franka_stage_parity(warm_steps, layout):
     rt = _rt()
     B = 128
-    model, dm, slab = _franka_slab(B, warm_steps)
+    model, dm, slab = _franka_slab(B, warm_steps,
+                                   layout=rt.LAYOUT_PLANES if layout == "planes" else rt.LAYOUT_ENV_RECORDS)
     oracle = Oracle(model, B, precision=32)
     for stage, outputs in [
         ("crba", ["mass_mat"]),
