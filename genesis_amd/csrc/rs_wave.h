@@ -68,6 +68,20 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Make every lane's earlier LDS writes visible to the other lanes of this wavefront, without waiting for
+// outstanding global-memory operations (so that prefetched loads stay in flight).
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// wave_sync for data exchanged through p's memory only: LDS-only when p points into LDS
+__device__ __forceinline__ void sync_for(const void* p) {
+    if (__builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)p)) lds_sync();
+    else wave_sync();
+}
+
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
 // exclusive prefix sum over the lanes of the wavefront (all lanes active)
@@ -111,7 +125,7 @@ template <int NQ>
 __device__ __forceinline__ void tile_seq_sums(float& acc, const float (&v)[NQ], int l0, int l1, float* tile, int lane) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) tile[q * kTileStride + lane] = v[q];
-    wave_sync();
+    sync_for(tile);
     if (lane < NQ) {
         const float* row = tile + lane * kTileStride;
         if (l0 == 0 && l1 == kWave) {
@@ -121,7 +135,7 @@ __device__ __forceinline__ void tile_seq_sums(float& acc, const float (&v)[NQ], 
             acc = seq_sum(acc, l0, l1, [&](int l) { return row[l]; });
         }
     }
-    wave_sync();
+    sync_for(tile);
 }
 
 // ---------------------------------------------------------------------------------------- private region

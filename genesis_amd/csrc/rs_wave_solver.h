@@ -344,15 +344,38 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const LState& 
     // quadratic terms (tile row D, times the ones of row D + 2, which leaves them exact), both in row order
     float q = lane == D ? cost[0] : 0.f;
     tile[(D + 2) * kTileStride + lane] = 1.f;
+    // this lane's row of the next chunk is loaded while the current chunk is summed
+    float n_ja = 0.f, n_f = 0.f, n_r = 0.f, n_D = 0.f, n_j[KD];
+    int n_act = 0;
+    auto load = [&](int c) {
+        if (c < nc) {
+            n_act = R.active(c);
+            n_ja = R.Jaref(c);
+            n_D = R.efc_D(c);
+            if (ne <= c && c < nef) {
+                n_f = R.fl(c);
+                n_r = R.diag(c);
+            }
+#pragma unroll
+            for (int d = 0; d < KD; d++)
+                if (d < D) n_j[d] = R.jac(c, d);
+        }
+    };
+    load(lane);
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
+        const int prev_act = n_act;
+        const float ja = n_ja, fl = n_f, dg = n_r, Dc = n_D;
+        float jr[KD];
+#pragma unroll
+        for (int d = 0; d < KD; d++) jr[d] = n_j[d];
+        load(c + kWave);
         if (c < nc) {
-            R.prev(c) = R.active(c);
+            R.prev(c) = prev_act;
             int act = 1;
             float ff = 0.f;
-            const float ja = R.Jaref(c);
             if (ne <= c && c < nef) {
-                const float f = R.fl(c), r = R.diag(c);
+                const float f = fl, r = dg;
                 const float rf = r * f;
                 const bool ln = ja <= -rf, lp = ja >= rf;
                 act = !(ln || lp);
@@ -361,21 +384,22 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const LState& 
                 act = ja < 0.f;
             }
             R.active(c) = act;
-            const float Dc = R.efc_D(c);
             const float force = ff + (-ja * Dc * (float)act);
             R.force(c) = force;
-            for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(c, d);
+#pragma unroll
+            for (int d = 0; d < KD; d++)
+                if (d < D) tile[d * kTileStride + lane] = jr[d];
             tile[D * kTileStride + lane] = 0.5f * (ja * ja * Dc * (float)act);
             tile[(D + 1) * kTileStride + lane] = force;
         }
         const int n = min(kWave, nc - c0);
-        wave_sync();
+        sync_for(tile);
         if (lane <= D) {
             const float* col = tile + lane * kTileStride;
             const float* fc = tile + (lane < D ? D + 1 : D + 2) * kTileStride;
-            q = seq_sum(q, 0, n, [&](int l) { return col[l] * fc[l]; });
+            q = seq_sum<16>(q, 0, n, [&](int l) { return col[l] * fc[l]; });
         }
-        wave_sync();
+        sync_for(tile);
     }
     if (lane < D) LF(qfrc_constraint, lane) = q;
     LF(gauss, 0) = gauss;
@@ -394,38 +418,73 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState
     const float eps = m.eps;
     constexpr int kE = (kTileMaxDofs * (kTileMaxDofs + 1) / 2 + kWave - 1) / kWave;  // entries per lane
     float h[kE];
+    int ei[kE], ej[kE];
 #pragma unroll
-    for (int r = 0; r < kE; r++) h[r] = 0.f;
+    for (int r = 0; r < kE; r++) {
+        h[r] = 0.f;
+        const int e = lane + r * kWave;
+        int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+        while ((i + 1) * (i + 2) / 2 <= e) i++;
+        while (i * (i + 1) / 2 > e) i--;
+        ei[r] = i;
+        ej[r] = e - i * (i + 1) / 2;
+    }
+    // this lane's row of the next chunk is loaded while the current chunk is summed
+    float n_j[kTileMaxDofs], n_D = 0.f;
+    int n_act = 0;
+    auto load = [&](int c) {
+        if (c < nc) {
+            n_D = R.efc_D(c);
+            n_act = R.active(c);
+#pragma unroll
+            for (int d = 0; d < kTileMaxDofs; d++)
+                if (d < D) n_j[d] = R.jac(c, d);
+        }
+    };
+    load(lane);
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
         if (c < nc) {
-            for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(c, d);
-            tile[D * kTileStride + lane] = R.efc_D(c);
-            tile[(D + 1) * kTileStride + lane] = (float)R.active(c);
+#pragma unroll
+            for (int d = 0; d < kTileMaxDofs; d++)
+                if (d < D) tile[d * kTileStride + lane] = n_j[d];
+            tile[D * kTileStride + lane] = n_D;
+            tile[(D + 1) * kTileStride + lane] = (float)n_act;
         }
+        load(c + kWave);
         const int n = min(kWave, nc - c0);
-        wave_sync();
+        sync_for(tile);
 #pragma unroll
         for (int r = 0; r < kE; r++) {
-            const int e = lane + r * kWave;
-            if (e < nE) {
-                int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
-                while ((i + 1) * (i + 2) / 2 <= e) i++;
-                while (i * (i + 1) / 2 > e) i--;
-                const int j = e - i * (i + 1) / 2;
-                const float* ci = tile + i * kTileStride;
-                const float* cj = tile + j * kTileStride;
+            if (lane + r * kWave < nE) {
+                const float* ci = tile + ei[r] * kTileStride;
+                const float* cj = tile + ej[r] * kTileStride;
                 const float* cd = tile + D * kTileStride;
                 const float* ca = tile + (D + 1) * kTileStride;
                 float acc = h[r];
-                for (int l = 0; l < n; l++) {
+                // terms in row order; the loads of 8 rows are issued before their adds
+                int l = 0;
+                for (; l + 8 <= n; l += 8) {
+                    float t[8];
+                    bool use[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const float ji = ci[l + u];
+                        use[u] = fabsf(ji) > eps;
+                        t[u] = cj[l + u] * ji * cd[l + u] * ca[l + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (use[u]) acc = acc + t[u];
+                }
+                for (; l < n; l++) {
                     const float ji = ci[l];
                     if (fabsf(ji) > eps) acc = acc + cj[l] * ji * cd[l] * ca[l];
                 }
                 h[r] = acc;
             }
         }
-        wave_sync();
+        sync_for(tile);
     }
 #pragma unroll
     for (int r = 0; r < kE; r++) {
@@ -529,7 +588,7 @@ __device__ inline bool wave_cholesky_incremental_r(const DModel& m, const LState
         if (rot && i == kk) LF(nt_H, kk * D + kk) = r;
         if (run) k++;
         busy = __popcll(wave_ballot(inslot && i == 0 && k < D));
-        wave_sync();
+        sync_for(FP(loc, nt_H));
     }
     wave_sync();
     return false;
@@ -596,19 +655,39 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
     // lanes 0-2 sum t[q] = (gauss, g1, g2)[q] + f[q] over all rows, lanes 3-5 e[q] over the equality rows, in row
     // order, from tile row q
     float acc = lane == 0 ? gauss : lane == 1 ? g1 : lane == 2 ? g2 : 0.f;
+    // this lane's row of the next chunk is loaded while the current chunk is summed
+    float n_ja = 0.f, n_f = 0.f, n_r = 0.f, n_D = 0.f, n_j[KD];
+    auto load = [&](int c) {
+        if (c < nc) {
+            n_ja = R.Jaref(c);
+            n_D = R.efc_D(c);
+            if (ne <= c && c < nef) {
+                n_f = R.fl(c);
+                n_r = R.diag(c);
+            }
+#pragma unroll
+            for (int d = 0; d < KD; d++)
+                if (d < D) n_j[d] = R.jac(c, d);
+        }
+    };
+    load(lane);
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
+        const float ja = n_ja, Dc = n_D, fl_c = n_f, dg = n_r;
+        float jr[KD];
+#pragma unroll
+        for (int d = 0; d < KD; d++) jr[d] = n_j[d];
+        load(c + kWave);
         if (c < nc) {
             float jv = 0.f;
 #pragma unroll
             for (int d = 0; d < KD; d++)
-                if (d < D) jv = jv + R.jac(c, d) * sr[d];
+                if (d < D) jv = jv + jr[d] * sr[d];
             R.jv(c) = jv;
-            const float ja = R.Jaref(c), Dc = R.efc_D(c);
             float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
             if (c < ne) {
             } else if (c < nef) {
-                const float fl = R.fl(c), r = R.diag(c), rf = r * fl;
+                const float fl = fl_c, r = dg, rf = r * fl;
                 const bool ln = ja <= -rf, lp = ja >= rf;
                 if (ln || lp) {
                     f0 = (float)ln * fl * (-0.5f * rf - ja) + (float)lp * fl * (-0.5f * rf + ja);
@@ -626,13 +705,13 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
             tile[2 * kTileStride + lane] = f2;
         }
         const int n = min(kWave, nc - c0);
-        wave_sync();
+        sync_for(tile);
         if (lane < 6) {
             const float* row = tile + (lane < 3 ? lane : lane - 3) * kTileStride;
             const int l1 = lane < 3 ? n : max(0, min(n, ne - c0));
-            acc = seq_sum(acc, 0, l1, [&](int l) { return row[l]; });
+            acc = seq_sum<16>(acc, 0, l1, [&](int l) { return row[l]; });
         }
-        wave_sync();
+        sync_for(tile);
     }
     const float t[3] = {readlane_f(acc, 0), readlane_f(acc, 1), readlane_f(acc, 2)};
     LF(eq_sum, 0) = readlane_f(acc, 3);
@@ -658,16 +737,33 @@ __device__ inline void wave_ls_eval_r(const DModel& m, const LState& loc, const 
     float A[N];
 #pragma unroll
     for (int k = 0; k < N; k++) A[k] = al[k];
-    for (int c0 = (ne / kWave) * kWave; c0 < nc; c0 += kWave) {
+    // this lane's row of the next chunk is loaded while the current chunk is summed
+    auto load = [&](int c, float& ja, float& jv, float& Dc, float& f, float& r) {
+        ja = jv = Dc = f = r = 0.f;
+        if (c >= ne && c < nc) {
+            ja = R.Jaref(c);
+            jv = R.jv(c);
+            Dc = R.efc_D(c);
+            if (c < nef) {
+                f = R.fl(c);
+                r = R.diag(c);
+            }
+        }
+    };
+    int c0 = (ne / kWave) * kWave;
+    float n_ja, n_jv, n_D, n_f, n_r;
+    load(c0 + lane, n_ja, n_jv, n_D, n_f, n_r);
+    for (; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
+        const float ja = n_ja, jv = n_jv, Dc = n_D, fl = n_f, dg = n_r;
+        load(c + kWave, n_ja, n_jv, n_D, n_f, n_r);
         float v[3 * N];
 #pragma unroll
         for (int q = 0; q < 3 * N; q++) v[q] = 0.f;
         if (c >= ne && c < nc) {
-            const float ja = R.Jaref(c), jv = R.jv(c), Dc = R.efc_D(c);
             const float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
             if (c < nef) {
-                const float f = R.fl(c), r = R.diag(c), rf = r * f;
+                const float f = fl, r = dg, rf = r * f;
 #pragma unroll
                 for (int k = 0; k < N; k++) {
                     const float x = ja + A[k] * jv;
