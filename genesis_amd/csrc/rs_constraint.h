@@ -80,10 +80,23 @@ __device__ inline int frictionloss_rows(const DModel& m, const S& st, int b, con
     return nfl;
 }
 
-// solver.py:580-685 add_collision_constraints for contact ic: rows start + 4 ic .. + 3
-template <class R, class S>
+// Where contact_rows accumulates a jacobian row: in place in the row store (RowJac), or in a staging column that is
+// copied to the row store once the row is complete (the fused kernel stages in LDS, so that the walk's
+// read-modify-writes do not go through global memory).
+struct RowJac {
+    template <class R>
+    __device__ __forceinline__ float& at(const R& rows, int c, int k) const {
+        return rows.jac(c, k);
+    }
+    template <class R>
+    __device__ __forceinline__ void flush(const R&, int, int) const {}
+};
+
+// solver.py:580-685 add_collision_constraints for contact ic: rows start + 4 ic .. + 3. The impedance depends on the
+// contact only and is evaluated once for its four rows.
+template <class R, class S, class J = RowJac>
 __device__ inline void contact_rows(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
-                                    int ic, int start) {
+                                    int ic, int start, const J& jacc = J()) {
     const int D = m.D;
     const size_t CB = (size_t)cs.B;
     int la = cs.contact_link_a[ic * CB + cb], lb = cs.contact_link_b[ic * CB + cb];
@@ -95,11 +108,12 @@ __device__ inline void contact_rows(const DModel& m, const S& st, int b, const D
     orthogonals(nrm, d1, d2);
     float invw = m.link_invweight[2 * la];
     if (lb > -1) invw = invw + m.link_invweight[2 * lb];
+    const float imp = impedance(sp, -pen);
     for (int i = 0; i < 4; i++) {
         f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
         f3 n = dd * fr - nrm;
         int c = start + ic * 4 + i;
-        for (int k = 0; k < D; k++) rows.jac(c, k) = 0.f;
+        for (int k = 0; k < D; k++) jacc.at(rows, c, k) = 0.f;
         float jq = 0.f;
         for (int ab = 0; ab < 2; ab++) {
             float sign = ab == 0 ? -1.f : 1.f;
@@ -111,13 +125,13 @@ __device__ inline void contact_rows(const DModel& m, const S& st, int b, const D
                     f3 vel = qrot(cv - cross3(tpos, ca), qident());
                     float jj = dot3(sign * vel, n);
                     jq = jq + jj * SF(vel, k);
-                    rows.jac(c, k) = rows.jac(c, k) + jj;
+                    jacc.at(rows, c, k) = jacc.at(rows, c, k) + jj;
                 }
                 link = m.link_parent[link];
             }
         }
-        float imp, aref;
-        imp_aref(sp, -pen, jq, -pen, imp, aref);
+        jacc.flush(rows, c, D);
+        const float aref = reference_acc(sp, imp, jq, -pen);
         float dg = invw + fr * fr * invw;
         dg *= 2.f * fr * fr * (1.f - imp) / imp;
         dg = fmaxf(dg, m.eps);

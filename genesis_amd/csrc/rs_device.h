@@ -281,19 +281,30 @@ __device__ __forceinline__ void orthogonals(f3 a, f3& o1, f3& o2) {
     o1 = cross3(b, a);
     o2 = b;
 }
+// impedance of a constraint at signed depth neg_pen (depends on the solver parameters and the depth only)
 template <class P>
-__device__ __forceinline__ void imp_aref(P p, float neg_pen, float vel, float pos, float& imp, float& aref) {
-    float tc = p[0], dr = p[1], dmin = p[2], dmax = p[3], width = p[4], mid = p[5], power = p[6];
+__device__ __forceinline__ float impedance(P p, float neg_pen) {
+    float dmin = p[2], dmax = p[3], width = p[4], mid = p[5], power = p[6];
     float x = fabsf(neg_pen) / width;
     float ia = (1.f / powf(mid, power - 1.f)) * powf(x, power);
     float ib = 1.f - (1.f / powf(1.f - mid, power - 1.f)) * powf(1.f - x, power);
     float iy = x < mid ? ia : ib;
-    imp = dmin + iy * (dmax - dmin);
+    float imp = dmin + iy * (dmax - dmin);
     imp = fminf(fmaxf(imp, dmin), dmax);
-    imp = x > 1.f ? dmax : imp;
+    return x > 1.f ? dmax : imp;
+}
+// reference acceleration of a constraint of impedance imp at velocity vel and position pos
+template <class P>
+__device__ __forceinline__ float reference_acc(P p, float imp, float vel, float pos) {
+    float tc = p[0], dr = p[1], dmax = p[3];
     float b = 2.f / (dmax * tc);
     float k = 1.f / (dmax * dmax * tc * tc * dr * dr);
-    aref = -b * vel - k * imp * pos;
+    return -b * vel - k * imp * pos;
+}
+template <class P>
+__device__ __forceinline__ void imp_aref(P p, float neg_pen, float vel, float pos, float& imp, float& aref) {
+    imp = impedance(p, neg_pen);
+    aref = reference_acc(p, imp, vel, pos);
 }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 

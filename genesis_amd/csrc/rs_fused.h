@@ -318,7 +318,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     }
     // ConstraintSolver.resolve
     if (!m.disable_constraint) {
-        wave_constraints(m, loc, st, b, R, lane);
+        wave_constraints(m, loc, st, b, R, lane, fa.tile_off >= 0 ? base + fa.tile_off : nullptr);
         if (fa.tile_off >= 0) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, base + fa.tile_off);
         else wave_solve<0>(m, loc, st, b, R, lane, nullptr);
     }

@@ -30,8 +30,10 @@ struct FusedArgs {
 };
 
 // Row tile: one chunk of 64 constraint rows staged column by column (column k at k * kTileStride), so that the
-// lanes that sum a column over the rows read it from LDS; the odd stride spreads the columns over the banks.
-constexpr int kTileStride = 65;
+// lanes that sum a column over the rows read it from LDS. The stride (68 floats) keeps every row 16-byte aligned, so
+// that a full row is read with 16-byte loads, and puts rows q and q + 1 four banks apart, so that up to 16 lanes
+// reading different rows at the same column, and 64 lanes writing one row, hit distinct banks.
+constexpr int kTileStride = 68;
 constexpr int kTileMaxDofs = 16;
 constexpr int kTileMinRows = 12;  // the ordered sums of the line search use up to 9 rows
 __host__ __device__ constexpr int tile_rows(int n_dofs) {
@@ -61,26 +63,18 @@ constexpr int kTmpArrays = 9;
 // ------------------------------------------------------------------------------------------- wave primitives
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
-// Make every lane's earlier LDS and global writes visible to the other lanes of this wavefront.
+// Make every lane's earlier LDS and global writes visible to the other lanes of this wavefront. The fences are at
+// wavefront scope: every exchange in the fused kernel stays inside one wavefront (an environment never shares data
+// with another wavefront), and the memory model orders a wavefront's own accesses without waiting for them to
+// complete, so this orders the code (no compiler motion across it) without stalling on outstanding loads and stores.
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Make every lane's earlier LDS writes visible to the other lanes of this wavefront, without waiting for
-// outstanding global-memory operations (so that prefetched loads stay in flight).
-__device__ __forceinline__ void lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// wave_sync for data exchanged through p's memory only: LDS-only when p points into LDS
-__device__ __forceinline__ void sync_for(const void* p) {
-    if (__builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)p)) lds_sync();
-    else wave_sync();
-}
+// wave_sync for data exchanged through p's memory (LDS or global: the same wavefront-scope ordering)
+__device__ __forceinline__ void sync_for(const void*) { wave_sync(); }
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
@@ -116,6 +110,42 @@ __device__ __forceinline__ float seq_sum(float acc, int i0, int i1, F f) {
     return acc;
 }
 
+// acc + row[0] + row[1] + ... + row[63], added in that order; row is 16-byte aligned and read 4 floats per load
+__device__ __forceinline__ float row_sum64(float acc, const float* row) {
+    float4 v[kWave / 4];
+#pragma unroll
+    for (int k = 0; k < kWave / 4; k++) v[k] = ((const float4*)row)[k];
+#pragma unroll
+    for (int k = 0; k < kWave / 4; k++) {
+        acc = acc + v[k].x;
+        acc = acc + v[k].y;
+        acc = acc + v[k].z;
+        acc = acc + v[k].w;
+    }
+    return acc;
+}
+
+// acc + a[0] * b[0] + a[1] * b[1] + ... + a[63] * b[63], in that order (each product rounded, then added)
+__device__ __forceinline__ float row_dot64(float acc, const float* a, const float* b) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {  // two halves of 32 columns bound the registers the loads take
+        float4 u[kWave / 8], v[kWave / 8];
+#pragma unroll
+        for (int k = 0; k < kWave / 8; k++) {
+            u[k] = ((const float4*)a)[h * kWave / 8 + k];
+            v[k] = ((const float4*)b)[h * kWave / 8 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < kWave / 8; k++) {
+            acc = acc + u[k].x * v[k].x;
+            acc = acc + u[k].y * v[k].y;
+            acc = acc + u[k].z * v[k].z;
+            acc = acc + u[k].w * v[k].w;
+        }
+    }
+    return acc;
+}
+
 // Ordered sums through the row tile. Lane l writes its row values v[q] to column l of tile row q; lane q (q < NQ) then
 // adds row q's columns l0, l0 + 1, ..., l1 - 1 in that order to its own accumulator acc: the additions of a sequential
 // loop over the rows, with the NQ chains running side by side in NQ lanes instead of as NQ x 64 broadcasts. The row
@@ -129,8 +159,7 @@ __device__ __forceinline__ void tile_seq_sums(float& acc, const float (&v)[NQ], 
     if (lane < NQ) {
         const float* row = tile + lane * kTileStride;
         if (l0 == 0 && l1 == kWave) {
-#pragma unroll
-            for (int l = 0; l < kWave; l++) acc = acc + row[l];
+            acc = row_sum64(acc, row);
         } else {
             acc = seq_sum(acc, l0, l1, [&](int l) { return row[l]; });
         }

@@ -31,16 +31,30 @@ struct WorkRows {
 
 #define LF(field, row) FP(loc, field)[(row)]
 
+// a lane's column of the row tile as the staging buffer of the contact jacobian rows it builds
+struct TileJac {
+    float* col;
+    __device__ __forceinline__ float& at(const WorkRows&, int, int k) const { return col[k * kTileStride]; }
+    __device__ __forceinline__ void flush(const WorkRows& rows, int c, int D) const {
+        for (int k = 0; k < D; k++) rows.jac(c, k) = col[k * kTileStride];
+    }
+};
+
 // solver.py:580-1286: equality, friction-loss and limit rows uniformly (few rows), contact rows one lane per contact
 __device__ inline void wave_constraints(const DModel& m, const LState& loc, const DState& st, int b, const WorkRows& R,
-                                        int lane) {
+                                        int lane, float* tile) {
     int nc = equality_rows(m, loc, 0, R);
     const int neq = nc;
     int nfl = frictionloss_rows(m, loc, 0, R, nc);
     nc += nfl;
     if (m.enable_collision) {
         const int ncont = LF(n_contacts, 0);
-        for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc);
+        if (tile) {
+            const TileJac jt{tile + lane};
+            for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc, jt);
+        } else {
+            for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc);
+        }
         nc += ncont * 4;
     }
     if (m.enable_joint_limit) nc = limit_rows(m, loc, 0, R, nc);
@@ -397,7 +411,8 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const LState& 
         if (lane <= D) {
             const float* col = tile + lane * kTileStride;
             const float* fc = tile + (lane < D ? D + 1 : D + 2) * kTileStride;
-            q = seq_sum<16>(q, 0, n, [&](int l) { return col[l] * fc[l]; });
+            if (n == kWave) q = row_dot64(q, col, fc);
+            else q = seq_sum<16>(q, 0, n, [&](int l) { return col[l] * fc[l]; });
         }
         sync_for(tile);
     }
@@ -462,16 +477,22 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState
                 const float* cd = tile + D * kTileStride;
                 const float* ca = tile + (D + 1) * kTileStride;
                 float acc = h[r];
-                // terms in row order; the loads of 8 rows are issued before their adds
+                // terms in row order; the loads of 8 rows (two 16-byte loads per tile row) come before their adds
                 int l = 0;
                 for (; l + 8 <= n; l += 8) {
                     float t[8];
                     bool use[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const float ji = ci[l + u];
-                        use[u] = fabsf(ji) > eps;
-                        t[u] = cj[l + u] * ji * cd[l + u] * ca[l + u];
+                    for (int hh = 0; hh < 2; hh++) {
+                        const float4 vi = ((const float4*)(ci + l))[hh], vj = ((const float4*)(cj + l))[hh];
+                        const float4 vd = ((const float4*)(cd + l))[hh], va = ((const float4*)(ca + l))[hh];
+                        const float ji[4] = {vi.x, vi.y, vi.z, vi.w}, jj[4] = {vj.x, vj.y, vj.z, vj.w};
+                        const float dd[4] = {vd.x, vd.y, vd.z, vd.w}, aa[4] = {va.x, va.y, va.z, va.w};
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            use[4 * hh + u] = fabsf(ji[u]) > eps;
+                            t[4 * hh + u] = jj[u] * ji[u] * dd[u] * aa[u];
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < 8; u++)
@@ -709,7 +730,8 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
         if (lane < 6) {
             const float* row = tile + (lane < 3 ? lane : lane - 3) * kTileStride;
             const int l1 = lane < 3 ? n : max(0, min(n, ne - c0));
-            acc = seq_sum<16>(acc, 0, l1, [&](int l) { return row[l]; });
+            if (l1 == kWave) acc = row_sum64(acc, row);
+            else acc = seq_sum<16>(acc, 0, l1, [&](int l) { return row[l]; });
         }
         sync_for(tile);
     }
