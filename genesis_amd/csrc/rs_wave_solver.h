@@ -561,21 +561,35 @@ __device__ inline bool wave_cholesky_incremental_r(const DModel& m, const LState
     int c0 = -kWave, act = 0;
     uint64_t flips = 0;
     int next = 0, busy = 0;
+    // this lane's row of the next chunk (flags, efc_D, jacobian row) is loaded one chunk ahead
+    int p_act = 0, p_prev = 0;
+    float p_D = 0.f, p_j[KD];
+    auto fetch = [&](int cl) {
+        p_act = p_prev = 0;
+        if (cl < nc) {
+            p_act = R.active(cl);
+            p_prev = R.prev(cl);
+            p_D = R.efc_D(cl);
+#pragma unroll
+            for (int d = 0; d < KD; d++)
+                if (d < D) p_j[d] = R.jac(cl, d);
+        }
+    };
+    fetch(lane);
     for (;;) {
         while (!flips && c0 + kWave < nc) {
             c0 += kWave;
-            const int cl = c0 + lane;
-            bool flip = false;
             wave_sync();  // earlier readers of the tile are done
-            if (cl < nc) {
-                act = R.active(cl);
-                flip = (act ^ R.prev(cl)) != 0;
-                if (flip) {
-                    const float sq = sqrtf(R.efc_D(cl));
-                    for (int d = 0; d < D; d++) tile[d * kTileStride + lane] = R.jac(cl, d) * sq;
-                }
+            act = p_act;
+            const bool flip = (p_act ^ p_prev) != 0;
+            if (flip) {
+                const float sq = sqrtf(p_D);
+#pragma unroll
+                for (int d = 0; d < KD; d++)
+                    if (d < D) tile[d * kTileStride + lane] = p_j[d] * sq;
             }
             flips = wave_ballot(flip);
+            fetch(c0 + kWave + lane);
             wave_sync();
         }
         if (flips && busy < S) {  // slots finish in the order they start: slot `next` is the idle one

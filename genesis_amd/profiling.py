@@ -7,6 +7,7 @@
   ``match``;
 * ``traffic(...)``: HBM bytes per launch of a kernel from a FETCH_SIZE run and a WRITE_SIZE run, each scaled by what
   the same counter reads for a streaming copy of known size in the same run (the calibration kernel);
+* ``valu(...)``: VALU busy share and lane utilisation of a kernel from one ``--pmc`` run of VALU_COUNTERS;
 * ``source_digest()``: digest of the kernel sources and build flags, recorded with a profile so that a report can tell
   whether the profile belongs to the kernels it is reporting on.
 
@@ -93,6 +94,23 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: int) -> dic
     return out
 
 
+VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "GRBM_GUI_ACTIVE")
+
+
+def valu(directory: str, kernel: str, simds: int, xcds: int = 8) -> dict:
+    """VALU figures of ``kernel`` from one ``--pmc`` run of VALU_COUNTERS (means per dispatch). GRBM_GUI_ACTIVE is
+    summed over the device's XCDs (it reads about xcds times the kernel's duration in cycles), so the device's busy
+    cycles are GRBM_GUI_ACTIVE / xcds:
+    valu_busy = SQ_ACTIVE_INST_VALU * 4 / (simds * GRBM_GUI_ACTIVE / xcds), the share of SIMD cycles issuing VALU work;
+    valu_lane_util = SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU), the share of lanes active in them."""
+    out = {"simds": simds, "xcds": xcds}
+    for c in VALU_COUNTERS:
+        out[c], out["dispatches"] = counter_per_dispatch(directory, c, kernel)
+    out["valu_busy"] = out["SQ_ACTIVE_INST_VALU"] * 4.0 / (simds * out["GRBM_GUI_ACTIVE"] / xcds)
+    out["valu_lane_util"] = out["SQ_THREAD_CYCLES_VALU"] / (64.0 * out["SQ_ACTIVE_INST_VALU"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
@@ -103,17 +121,21 @@ def main():
     ap.add_argument("--bench-json", default=None, help="bench.py output of the traced run")
     ap.add_argument("--title", default="")
     ap.add_argument("--commands", default="", help="text of the commands that produced the runs")
+    ap.add_argument("--valu", default=None, help="directory of a --pmc run of VALU_COUNTERS")
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the device (MI355X: 256 CUs x 4)")
+    ap.add_argument("--xcds", type=int, default=8, help="XCDs of the device (MI355X: 8)")
     ap.add_argument("--out", required=True, help="output path without extension")
     a = ap.parse_args()
     stats = kernel_stats(a.trace)
     tr = traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
+    vu = valu(a.valu, a.kernel, a.simds, a.xcds) if a.valu else None
     k = next(s for s in stats if a.kernel in s["name"])
     bench = None
     if a.bench_json:
         with open(a.bench_json) as f:
             lines = [ln for ln in f if ln.startswith("{")]
         bench = json.loads(lines[-1]) if lines else None
-    fig = {"kernel": a.kernel, "avg_us": k["avg_us"], "calls": k["calls"], "traffic": tr,
+    fig = {"kernel": a.kernel, "avg_us": k["avg_us"], "calls": k["calls"], "traffic": tr, "valu": vu,
            "source_digest": source_digest(), "title": a.title}
     if bench:
         rf = bench["roofline"]
@@ -137,6 +159,13 @@ def main():
                      f"({t['copy_dispatches']} dispatches) -> factor {t['factor']:.4f} -> "
                      f"{t['bytes_per_launch'] / 1e6:.3f} MB/launch")
     lines.append(f"  traffic: {tr['bytes_per_launch'] / 1e6:.3f} MB per launch")
+    if vu:
+        lines += ["", f"VALU of {a.kernel} (--pmc {' '.join(VALU_COUNTERS)}; means per dispatch, {vu['simds']} SIMDs, "
+                      f"{vu['xcds']} XCDs):",
+                  "", "  " + ", ".join(f"{c} {vu[c]:.4g}" for c in VALU_COUNTERS),
+                  f"  VALU busy (SQ_ACTIVE_INST_VALU * 4 / (SIMDs * GRBM_GUI_ACTIVE / XCDs)): {100 * vu['valu_busy']:.1f}%",
+                  f"  VALU lane utilisation (SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU)): "
+                  f"{100 * vu['valu_lane_util']:.1f}%"]
     if bench:
         alg = fig["algorithmic_bytes_per_launch"]
         lines += ["", f"Algorithmic bytes per launch: {rf['bytes_per_env_step']:.1f} B/env-step x "

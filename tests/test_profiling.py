@@ -47,3 +47,18 @@ def test_kernel_stats_table(tmp_path):
 
 def test_source_digest_is_stable():
     assert profiling.source_digest() == profiling.source_digest()
+
+
+def test_valu_figures(tmp_path):
+    d = tmp_path / "v"
+    os.makedirs(d)
+    vals = {"SQ_INSTS_VALU": 1000.0, "SQ_ACTIVE_INST_VALU": 400.0, "SQ_THREAD_CYCLES_VALU": 12800.0,
+            "SQ_WAVE_CYCLES": 9000.0, "GRBM_GUI_ACTIVE": 100.0}
+    with open(d / "pmc_counter_collection.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(COLS)
+        for i, (c, v) in enumerate(vals.items()):
+            w.writerow([0, 0, 1, "rs::k_fused_wave<true, 0>", c, v])
+    v = profiling.valu(str(d), "k_fused_wave", simds=8, xcds=2)
+    assert abs(v["valu_busy"] - 400.0 * 4 / (8 * 100.0 / 2)) < 1e-12
+    assert abs(v["valu_lane_util"] - 12800.0 / (64 * 400.0)) < 1e-12
