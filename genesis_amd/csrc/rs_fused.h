@@ -288,12 +288,16 @@ __device__ inline void wave_narrowphase(const DModel& m, const LState& loc, cons
 }
 
 // One environment's substep on the calling wavefront (w = wavefront index in the workgroup).
-template <bool kLds>
+// kPath: 1 = models with a row tile (at most kTileMaxDofs dofs), 0 = the others, 2 = either (decided per call: the
+// shared launch of several models).
+template <bool kLds, int kPath = 2>
 __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, const FusedArgs& fa, int b, int w,
                                           float* lds_env) {
     const int lane = lane_id();
     float* wk = st.wk + (size_t)b * fa.work_stride;
-    float* base = kLds ? lds_env + (size_t)w * fa.loc_floats : wk + fa.off_loc;
+    float* slice = lds_env + (size_t)w * fa.lds_wave;
+    float* base = kLds ? slice : wk + fa.off_loc;
+    float* tile = slice + fa.tile_off;
     const LState loc = make_local(base, fa);
     const WorkRows R{wk + fa.off_jac, wk + fa.off_rowf, wk + fa.off_tmp, m.CMAX, m.D};
 
@@ -318,9 +322,11 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     }
     // ConstraintSolver.resolve
     if (!m.disable_constraint) {
-        wave_constraints(m, loc, st, b, R, lane, fa.tile_off >= 0 ? base + fa.tile_off : nullptr);
-        if (fa.tile_off >= 0) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, base + fa.tile_off);
-        else wave_solve<0>(m, loc, st, b, R, lane, nullptr);
+        wave_constraints(m, loc, st, b, R, lane, fa.tiled ? tile : nullptr);
+        if constexpr (kPath == 1) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, tile);
+        else if constexpr (kPath == 0) wave_solve<0>(m, loc, st, b, R, lane, tile);
+        else if (fa.tiled) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, tile);
+        else wave_solve<0>(m, loc, st, b, R, lane, tile);
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
     wave_update_acc(m, loc, lane, true);
@@ -346,7 +352,7 @@ __global__ void k_sched_flip(DState st) {
 
 // kOcc > 0 asks the compiler for at least kOcc wavefronts per SIMD (fewer registers, more spills); 0 leaves the
 // register budget to the compiler. The autotuner picks between them.
-template <bool kLds, int kOcc>
+template <bool kLds, int kOcc, int kPath>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kOcc > 0 ? kOcc : 1, 8)))
 k_fused_wave(DModel m, DState st, FusedArgs fa) {
     extern __shared__ float lds_env[];
@@ -374,7 +380,7 @@ k_fused_wave(DModel m, DState st, FusedArgs fa) {
         }
     }
     if (b >= st.B) return;  // whole wavefronts only
-    fused_env<kLds>(m, st, fa, b, w, lds_env);
+    fused_env<kLds, kPath>(m, st, fa, b, w, lds_env);
     if (fa.sched && lane_id() == 0) {
         const int cls = sched_class(st.n_con[b]);
         const int q = par ^ 1;

@@ -5,6 +5,7 @@
 // Fused path: the whole substep in a single kernel per launch, one wavefront per environment (rs_fused.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -129,12 +130,10 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     }
     fa->n_local = (int)off;
     off = up4(off);
-    fa->tile_off = -1;
-    if (h.n_dofs <= kTileMaxDofs) {
-        fa->tile_off = (int)off;
-        off += up4((int64_t)kTileStride * tile_rows(h.n_dofs));
-    }
     fa->loc_floats = (int)off;
+    fa->tiled = h.n_dofs <= kTileMaxDofs;
+    fa->tile_off = 0;  // set per launch by set_lds_slice
+    fa->lds_wave = (int)up4((int64_t)kTileStride * tile_rows(h.n_dofs));
     fa->sched = 0;
     const int64_t C = h.len_constraints > 0 ? h.len_constraints : 1, D = h.n_dofs;
     fa->off_loc = 0;
@@ -145,6 +144,17 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->off_mc = (int)(fa->off_pc + up4((int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8));
     int64_t total = fa->off_mc + kMcFloats;
     fa->work_stride = (total + 63) / 64 * 64;
+}
+
+// LDS slice of one wavefront: the private region (when it is in LDS) followed by the row tile
+static int tile_floats(const rs_model_header& h) { return (kTileStride * tile_rows(h.n_dofs) + 3) / 4 * 4; }
+static void set_lds_slice(const rs_model_header& h, FusedArgs* fa, bool use_lds, int slice_floats) {
+    fa->tile_off = use_lds ? fa->loc_floats : 0;
+    fa->lds_wave = slice_floats;
+    (void)h;
+}
+static int lds_slice_floats(const rs_model_header& h, const FusedArgs& fa, bool use_lds) {
+    return (use_lds ? fa.loc_floats : 0) + tile_floats(h);
 }
 
 static int64_t work_floats(const rs_model_header& h) {
@@ -319,15 +329,23 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     fa.sched = m->schedule && st.B >= sched_rows();
     if (fa.sched) hipLaunchKernelGGL(k_sched_flip, dim3(1), dim3(1), 0, s, st);
     const dim3 grid((unsigned)((st.B + epb - 1) / epb)), block((unsigned)(kWave * epb));
-    const size_t lds = (size_t)epb * fa.loc_floats * 4;
-    if (m->use_lds) {
-        if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<true, 4>), grid, block, lds, s, m->dm, st, fa);
-        else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3>), grid, block, lds, s, m->dm, st, fa);
-        else hipLaunchKernelGGL((k_fused_wave<true, 0>), grid, block, lds, s, m->dm, st, fa);
+    set_lds_slice(m->h, &fa, m->use_lds, lds_slice_floats(m->h, fa, m->use_lds));
+    const size_t lds = (size_t)epb * fa.lds_wave * 4;
+    // one kernel variant per (private region, occupancy) for models with a row tile; models without one run the
+    // generic-solver variants at the compiler's register budget
+    if (fa.tiled) {
+        if (m->use_lds) {
+            if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<true, 4, 1>), grid, block, lds, s, m->dm, st, fa);
+            else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3, 1>), grid, block, lds, s, m->dm, st, fa);
+            else hipLaunchKernelGGL((k_fused_wave<true, 0, 1>), grid, block, lds, s, m->dm, st, fa);
+        } else {
+            if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<false, 4, 1>), grid, block, lds, s, m->dm, st, fa);
+            else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3, 1>), grid, block, lds, s, m->dm, st, fa);
+            else hipLaunchKernelGGL((k_fused_wave<false, 0, 1>), grid, block, lds, s, m->dm, st, fa);
+        }
     } else {
-        if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<false, 4>), grid, block, 0, s, m->dm, st, fa);
-        else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3>), grid, block, 0, s, m->dm, st, fa);
-        else hipLaunchKernelGGL((k_fused_wave<false, 0>), grid, block, 0, s, m->dm, st, fa);
+        if (m->use_lds) hipLaunchKernelGGL((k_fused_wave<true, 0, 0>), grid, block, lds, s, m->dm, st, fa);
+        else hipLaunchKernelGGL((k_fused_wave<false, 0, 0>), grid, block, lds, s, m->dm, st, fa);
     }
     HIP_OK(hipGetLastError());
     return RS_OK;
@@ -335,8 +353,9 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
 
 static int check_fused_config(const rs_model* m, int epb, int use_lds) {
     if (epb < 1 || epb > 8) return fail(RS_E_INVALID, "fused config: envs_per_block must be in 1..8");
-    if (use_lds && (int64_t)epb * m->fa.loc_floats * 4 > m->max_lds)
-        return fail(RS_E_UNSUPPORTED, "fused config: private regions do not fit in LDS at this envs_per_block");
+    if ((int64_t)epb * lds_slice_floats(m->h, m->fa, use_lds) * 4 > m->max_lds)
+        return fail(RS_E_UNSUPPORTED, use_lds ? "fused config: private regions do not fit in LDS at this envs_per_block"
+                                              : "fused config: row tiles do not fit in LDS at this envs_per_block");
     return RS_OK;
 }
 
@@ -448,7 +467,7 @@ int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
     if (!m || !out) return fail(RS_E_INVALID, "rs_fused_config_get: null argument");
     out->envs_per_block = m->fa.epb;
     out->use_lds = m->use_lds;
-    out->lds_bytes_per_env = m->fa.loc_floats * 4;
+    out->lds_bytes_per_env = lds_slice_floats(m->h, m->fa, m->use_lds) * 4;
     out->schedule = m->schedule;
     out->occupancy = m->occupancy;
     return RS_OK;
@@ -457,6 +476,8 @@ int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
 int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     if (!m || !c) return fail(RS_E_INVALID, "rs_fused_config_set: null argument");
     if (int r = check_fused_config(m, c->envs_per_block, c->use_lds != 0)) return r;
+    if (!m->fa.tiled && c->occupancy != 0)
+        return fail(RS_E_UNSUPPORTED, "fused config: occupancy variants exist for models with a row tile only");
     m->fa.epb = c->envs_per_block;
     m->use_lds = c->use_lds != 0;
     m->schedule = c->schedule != 0;
@@ -629,14 +650,16 @@ int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* cons
         return fail(RS_E_INVALID, "rs_batch_step: bad argument");
     hipStream_t s = (hipStream_t)stream;
     const int epb = 2;
-    int max_loc = 0;
+    int slice_lds = 0, slice_hbm = 0;
     for (int i = 0; i < n; i++) {
         if (!models[i] || !states[i] || n_envs[i] < 1) return fail(RS_E_INVALID, "rs_batch_step: bad segment");
-        max_loc = max_loc > models[i]->fa.loc_floats ? max_loc : models[i]->fa.loc_floats;
+        slice_lds = std::max(slice_lds, lds_slice_floats(models[i]->h, models[i]->fa, true));
+        slice_hbm = std::max(slice_hbm, lds_slice_floats(models[i]->h, models[i]->fa, false));
     }
     int max_lds = models[0]->max_lds;
     for (int i = 1; i < n; i++) max_lds = max_lds < models[i]->max_lds ? max_lds : models[i]->max_lds;
-    const bool use_lds = (int64_t)epb * max_loc * 4 <= max_lds;
+    const bool use_lds = (int64_t)epb * slice_lds * 4 <= max_lds;
+    const int slice = use_lds ? slice_lds : slice_hbm;
     const int k = bt->slot;
     bt->slot = (bt->slot + 1) % rs_batch::kRing;
     if (bt->pending[k]) HIP_OK(hipEventSynchronize(bt->done[k]));
@@ -646,6 +669,7 @@ int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* cons
         tab[i].m = models[i]->dm;
         tab[i].st = make_state(models[i], states[i], n_envs[i]);
         tab[i].fa = models[i]->fa;
+        set_lds_slice(models[i]->h, &tab[i].fa, use_lds, slice);
         tab[i].fa.epb = epb;
         tab[i].fa.sched = 0;
         tab[i].first_block = blocks;
@@ -657,9 +681,9 @@ int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* cons
     bt->pending[k] = true;
     const dim3 grid((unsigned)blocks), block((unsigned)(kWave * epb));
     if (use_lds)
-        hipLaunchKernelGGL(k_fused_multi<true>, grid, block, (size_t)epb * max_loc * 4, s, bt->dev[k], n);
+        hipLaunchKernelGGL(k_fused_multi<true>, grid, block, (size_t)epb * slice * 4, s, bt->dev[k], n);
     else
-        hipLaunchKernelGGL(k_fused_multi<false>, grid, block, 0, s, bt->dev[k], n);
+        hipLaunchKernelGGL(k_fused_multi<false>, grid, block, (size_t)epb * slice * 4, s, bt->dev[k], n);
     HIP_OK(hipGetLastError());
     return RS_OK;
 }

@@ -22,9 +22,11 @@ struct FusedArgs {
     int epb;           // environments (wavefronts) per workgroup
     int n_local;       // local rows staged per environment (the packed local planes)
     int loc_floats;    // private region per environment (n_local rounded up to 16 bytes)
+    int tiled;         // 1: the model has at most kTileMaxDofs dofs (the row-tile solver variants apply)
+    int lds_wave;      // floats of LDS per wavefront: [private region if in LDS][row tile]
     int64_t work_stride;  // floats of the work field per environment
     int off_loc, off_jac, off_rowf, off_tmp, off_pc, off_mc;
-    int tile_off;      // row tile in the private region (kTileStride x tile_rows(D) floats), -1 when D > kTileMaxDofs
+    int tile_off;      // offset of the row tile (kTileStride x tile_rows(D) floats) in the wavefront's LDS slice
     int sched;         // 1: map wavefronts to environments through the cost-ordered queues
     int loc_off[kMaxStateFields];  // private-region offset of each state field (local kinds only)
 };
@@ -37,7 +39,7 @@ constexpr int kTileStride = 68;
 constexpr int kTileMaxDofs = 16;
 constexpr int kTileMinRows = 12;  // the ordered sums of the line search use up to 9 rows
 __host__ __device__ constexpr int tile_rows(int n_dofs) {
-    return n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;
+    return n_dofs <= kTileMaxDofs && n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;
 }
 
 // Schedule: after its substep each environment appends itself to one of kSchedClasses queues by cost (its number of

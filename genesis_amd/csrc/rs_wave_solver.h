@@ -109,6 +109,49 @@ __device__ inline float wave_update_constraint(const DModel& m, const LState& lo
     return cost;
 }
 
+// solver.py:1595-1621 func_cholesky_factor_direct_batch on the lower triangle of the D x D matrix H (row-major), in
+// place: column i's diagonal first (one chain on every lane), then the entries below it one lane per row, each sum
+// over k taken in order. The terms' loads are issued 8 at a time ahead of their adds, since H may live in global
+// memory.
+__device__ inline void cholesky_in_place(int D, float* H, float eps, int lane) {
+    constexpr int U = 8;
+    for (int i = 0; i < D; i++) {
+        const float* Hi = H + (size_t)i * D;
+        float t = Hi[i];
+        int j = 0;
+        for (; j + U <= i; j += U) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) v[u] = Hi[j + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) t = t - v[u] * v[u];
+        }
+        for (; j < i; j++) t = t - Hi[j] * Hi[j];
+        const float dii = sqrtf(fmaxf(t, eps));
+        const float inv = 1.f / dii;
+        for (int r = i + 1 + lane; r < D; r += kWave) {
+            const float* Hr = H + (size_t)r * D;
+            float dt = 0.f;
+            int k = 0;
+            for (; k + U <= i; k += U) {
+                float a[U], c[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    a[u] = Hr[k + u];
+                    c[u] = Hi[k + u];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) dt = dt + a[u] * c[u];
+            }
+            for (; k < i; k++) dt = dt + Hr[k] * Hi[k];
+            H[(size_t)r * D + i] = (Hr[i] - dt) * inv;
+        }
+        wave_sync();
+        H[(size_t)i * D + i] = dii;
+        wave_sync();
+    }
+}
+
 // solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
 __device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
@@ -144,20 +187,7 @@ __device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc,
         LF(nt_H, i * D + j) = h;
     }
     wave_sync();
-    for (int i = 0; i < D; i++) {
-        float t = LF(nt_H, i * D + i);
-        for (int j = 0; j < i; j++) t = t - LF(nt_H, i * D + j) * LF(nt_H, i * D + j);
-        float dii = sqrtf(fmaxf(t, eps));
-        float inv = 1.f / dii;
-        for (int j = i + 1 + lane; j < D; j += kWave) {
-            float dt = 0.f;
-            for (int k = 0; k < i; k++) dt = dt + LF(nt_H, j * D + k) * LF(nt_H, i * D + k);
-            LF(nt_H, j * D + i) = (LF(nt_H, j * D + i) - dt) * inv;
-        }
-        wave_sync();
-        LF(nt_H, i * D + i) = dii;
-    }
-    wave_sync();
+    cholesky_in_place(D, FP(loc, nt_H), eps, lane);
 }
 
 // solver.py:1782-1822: rank-1 updates of the factor for rows whose active flag flipped, in row order;
@@ -214,122 +244,117 @@ __device__ inline void wave_gradient(const DModel& m, const LState& loc) {
     }
 }
 
-// solver.py:2034-2152 func_ls_init_and_eval_p0_opt
-__device__ inline LsP wave_ls_init(const DModel& m, const LState& loc, const WorkRows& R, int lane, int& ls_it) {
+// solver.py:2722-2751 + :1893-1910 for models with up to 128 dofs, lane-parallel: lane l holds rows l and l + 64.
+// Forward substitution column by column (row i subtracts L[i][j] x_j for j = 0, 1, ... in turn, the reference's
+// order for row i); backward substitution row by row, row i subtracting L[j][i] x_j for j = i + 1, i + 2, ... in
+// turn, with each product formed by lane j and the subtractions chained through readlane. The factor may live in
+// global memory (large models keep their private region there), so its columns are loaded kAhead steps early.
+__device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, int lane) {
+    constexpr int kAhead = 4;
     const int D = m.D;
-    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
-    for (int i = lane; i < D; i += kWave) {
-        const int e = m.dof_entity[i];
-        float mv = 0.f;
-        for (int j = m.entity_dof_start[e]; j < m.entity_dof_end[e]; j++) mv = mv + LF(mass_mat, i * D + j) * LF(search, j);
-        LF(mv, i) = mv;
-    }
-    for (int c = lane; c < nc; c += kWave) {
-        const float jv = seq_sum(0.f, 0, D, [&](int d) { return R.jac(c, d) * LF(search, d); });
-        R.jv(c) = jv;
-        float ja = R.Jaref(c), Dc = R.efc_D(c);
-        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
-        if (c < ne) {
-        } else if (c < nef) {
-            float f = R.fl(c), r = R.diag(c), rf = r * f;
-            bool ln = ja <= -rf, lp = ja >= rf;
-            if (ln || lp) {
-                f0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
-                f1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
-                f2 = 0.f;
-            }
-        } else {
-            float a = (float)(ja < 0.f);
-            f0 = f0 * a;
-            f1 = f1 * a;
-            f2 = f2 * a;
+    const float* L = FP(loc, nt_H);
+    float cur[2], hd[2], x[2] = {0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int i = lane + r * kWave;
+        cur[r] = 0.f;
+        hd[r] = 1.f;
+        if (i < D) {
+            const float g = LF(Ma, i) - LF(force, i) - LF(qfrc_constraint, i);
+            LF(grad, i) = g;
+            cur[r] = g;
+            hd[r] = L[i * D + i];
         }
-        R.tmp(0, c) = f0;
-        R.tmp(1, c) = f1;
-        R.tmp(2, c) = f2;
     }
+    // forward: pf[k][r] = L[row r of this lane][column j0 + k]
+    auto colv = [&](int i, int j) { return (j >= 0 && i < D && i > j) ? L[i * D + j] : 0.f; };
+    float pf[kAhead][2];
+#pragma unroll
+    for (int k = 0; k < kAhead; k++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) pf[k][r] = colv(lane + r * kWave, k);
+    for (int j0 = 0; j0 < D; j0 += kAhead) {
+#pragma unroll
+        for (int k = 0; k < kAhead; k++) {
+            const int j = j0 + k;
+            if (j < D) {
+                const int rj = j >> 6, lj = j & (kWave - 1);
+                const float xj = readlane_f(rj ? cur[1] : cur[0], lj) / readlane_f(rj ? hd[1] : hd[0], lj);
+                if (lane == lj) {
+                    if (rj) x[1] = xj;
+                    else x[0] = xj;
+                }
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const int i = lane + r * kWave;
+                    if (i < D && i > j) cur[r] = cur[r] - pf[k][r] * xj;
+                    pf[k][r] = colv(i, j + kAhead);
+                }
+            }
+        }
+    }
+    // backward: pb[k][r] = L[row r of this lane][column i], for i = D - 1 - (ii0 + k)
+    float pb[kAhead][2];
+#pragma unroll
+    for (int k = 0; k < kAhead; k++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) pb[k][r] = colv(lane + r * kWave, D - 1 - k);
+    for (int ii0 = 0; ii0 < D; ii0 += kAhead) {
+#pragma unroll
+        for (int k = 0; k < kAhead; k++) {
+            const int i = D - 1 - (ii0 + k);
+            if (i >= 0) {
+                float prod[2];
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    prod[r] = pb[k][r] * x[r];  // lane j: L[j][i] x_j (zero unless j > i)
+                    pb[k][r] = colv(lane + r * kWave, i - kAhead);
+                }
+                const int ri = i >> 6, li = i & (kWave - 1);
+                float c = readlane_f(ri ? x[1] : x[0], li);
+                // j = i + 1 .. D - 1 in order: first the rows held in prod[0] (j < 64), then those in prod[1]
+                const int j1 = min(D, kWave);
+                int j = i + 1;
+                for (; j + 4 <= j1; j += 4) {
+                    const float t0 = readlane_f(prod[0], j), t1 = readlane_f(prod[0], j + 1);
+                    const float t2 = readlane_f(prod[0], j + 2), t3 = readlane_f(prod[0], j + 3);
+                    c = c - t0;
+                    c = c - t1;
+                    c = c - t2;
+                    c = c - t3;
+                }
+                for (; j < j1; j++) c = c - readlane_f(prod[0], j);
+                j = max(j, kWave);
+                for (; j + 4 <= D; j += 4) {
+                    const float t0 = readlane_f(prod[1], j - kWave), t1 = readlane_f(prod[1], j - kWave + 1);
+                    const float t2 = readlane_f(prod[1], j - kWave + 2), t3 = readlane_f(prod[1], j - kWave + 3);
+                    c = c - t0;
+                    c = c - t1;
+                    c = c - t2;
+                    c = c - t3;
+                }
+                for (; j < D; j++) c = c - readlane_f(prod[1], j - kWave);
+                c = c / readlane_f(ri ? hd[1] : hd[0], li);
+                if (lane == li) {
+                    if (ri) x[1] = c;
+                    else x[0] = c;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+        if (lane + r * kWave < D) LF(Mgrad, lane + r * kWave) = x[r];
     wave_sync();
-    float g1 = 0.f, g2 = 0.f;
-    for (int d = 0; d < D; d++) {
-        float s = LF(search, d);
-        g1 = g1 + (s * LF(Ma, d) - s * LF(force, d));
-        g2 = g2 + 0.5f * s * LF(mv, d);
-    }
-    float gauss = LF(gauss, 0);
-    LF(quad_gauss, 0) = gauss;
-    LF(quad_gauss, 1) = g1;
-    LF(quad_gauss, 2) = g2;
-    float t0 = gauss, t1 = g1, t2 = g2, e0 = 0.f, e1 = 0.f, e2 = 0.f;
-    for (int c = 0; c < ne; c++) {
-        float f0 = R.tmp(0, c), f1 = R.tmp(1, c), f2 = R.tmp(2, c);
-        e0 = e0 + f0; e1 = e1 + f1; e2 = e2 + f2;
-        t0 = t0 + f0; t1 = t1 + f1; t2 = t2 + f2;
-    }
-    t0 = seq_sum(t0, ne, nc, [&](int c) { return R.tmp(0, c); });
-    t1 = seq_sum(t1, ne, nc, [&](int c) { return R.tmp(1, c); });
-    t2 = seq_sum(t2, ne, nc, [&](int c) { return R.tmp(2, c); });
-    LF(eq_sum, 0) = e0;
-    LF(eq_sum, 1) = e1;
-    LF(eq_sum, 2) = e2;
-    float h = 2.f * t2;
-    if (h <= 0.f) h = m.eps;
-    ls_it = 1;
-    wave_sync();
-    return {0.f, t0, t1, h};
 }
 
-// solver.py:2155-2355 (n = 1 or 3 candidate steps in one pass over the rows)
-__device__ inline void wave_ls_eval(const DModel& m, const LState& loc, const WorkRows& R, int lane, const float* al,
-                                    int n, LsP* out, int& ls_it) {
-    const int nc = LF(n_con, 0), ne = LF(n_con_eq, 0), nef = ne + LF(n_con_fl, 0);
-    for (int c = ne + lane; c < nc; c += kWave) {
-        float ja = R.Jaref(c), jv = R.jv(c), Dc = R.efc_D(c);
-        float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
-        if (c < nef) {
-            float f = R.fl(c), r = R.diag(c), rf = r * f;
-            for (int k = 0; k < n; k++) {
-                float x = ja + al[k] * jv;
-                bool ln = x <= -rf, lp = x >= rf;
-                float a0 = f0, a1 = f1, a2 = f2;
-                if (ln || lp) {
-                    a0 = (float)ln * f * (-0.5f * rf - ja) + (float)lp * f * (-0.5f * rf + ja);
-                    a1 = (float)ln * (-f * jv) + (float)lp * (f * jv);
-                    a2 = 0.f;
-                }
-                R.tmp(3 * k, c) = a0;
-                R.tmp(3 * k + 1, c) = a1;
-                R.tmp(3 * k + 2, c) = a2;
-            }
-        } else {
-            for (int k = 0; k < n; k++) {
-                float a = (float)(ja + al[k] * jv < 0.f);
-                R.tmp(3 * k, c) = f0 * a;
-                R.tmp(3 * k + 1, c) = f1 * a;
-                R.tmp(3 * k + 2, c) = f2 * a;
-            }
-        }
-    }
-    wave_sync();
-    const float b0 = LF(quad_gauss, 0) + LF(eq_sum, 0), b1 = LF(quad_gauss, 1) + LF(eq_sum, 1),
-                b2 = LF(quad_gauss, 2) + LF(eq_sum, 2);
-    for (int k = 0; k < n; k++) {
-        const float t0 = seq_sum(b0, ne, nc, [&](int c) { return R.tmp(3 * k, c); });
-        const float t1 = seq_sum(b1, ne, nc, [&](int c) { return R.tmp(3 * k + 1, c); });
-        const float t2 = seq_sum(b2, ne, nc, [&](int c) { return R.tmp(3 * k + 2, c); });
-        float a = al[k];
-        float h = 2.f * t2;
-        if (h <= 0.f) h = m.eps;
-        out[k] = {a, a * a * t2 + a * t1 + t0, 2.f * a * t2 + t1, h};
-    }
-    ls_it += n;
-    wave_sync();
-}
 
 // ------------------------------------------------------------------------------------------------------------
-// Variants for models with at most KD dofs (instantiated for KD = 16), which have a row tile in the private region:
-// each lane computes its row's terms in registers and stages them in the tile, and every sum over rows runs in one
-// lane per output, in row order (tile_seq_sums), so many ordered sums proceed side by side. Same arithmetic and order
-// as the functions above.
+// Row-tile variants. Every fused launch gives each wavefront a row tile in LDS; each lane computes its row's terms
+// in registers and stages them in the tile, and every sum over rows runs in one lane per output, in row order
+// (tile_seq_sums), so many ordered sums proceed side by side. The line-search passes serve every model (KD = 0:
+// any number of dofs); the constraint update, Hessian, rank-1 update and triangular-solve variants need at most KD
+// dofs (instantiated for KD = 16). Same arithmetic and order as the functions above.
 
 // solver.py:2620-2694
 template <int KD>
@@ -521,20 +546,7 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState
         }
     }
     wave_sync();
-    for (int i = 0; i < D; i++) {
-        float t = LF(nt_H, i * D + i);
-        for (int j = 0; j < i; j++) t = t - LF(nt_H, i * D + j) * LF(nt_H, i * D + j);
-        float dii = sqrtf(fmaxf(t, eps));
-        float inv = 1.f / dii;
-        for (int j = i + 1 + lane; j < D; j += kWave) {
-            float dt = 0.f;
-            for (int k = 0; k < i; k++) dt = dt + LF(nt_H, j * D + k) * LF(nt_H, i * D + k);
-            LF(nt_H, j * D + i) = (LF(nt_H, j * D + i) - dt) * inv;
-        }
-        wave_sync();
-        LF(nt_H, i * D + i) = dii;
-    }
-    wave_sync();
+    cholesky_in_place(D, FP(loc, nt_H), eps, lane);
 }
 
 // solver.py:1782-1822 with the rank-1 updates of consecutive flipped rows pipelined. Update u (the u-th flipped row in
@@ -673,9 +685,10 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
         for (int j = m.entity_dof_start[e]; j < m.entity_dof_end[e]; j++) mv = mv + LF(mass_mat, i * D + j) * LF(search, j);
         LF(mv, i) = mv;
     }
-    float sr[KD];
+    constexpr int KR = KD > 0 ? KD : 1;  // register copies of the search direction and of a jacobian row (KD > 0)
+    float sr[KR];
 #pragma unroll
-    for (int d = 0; d < KD; d++) sr[d] = d < D ? LF(search, d) : 0.f;
+    for (int d = 0; d < KR; d++) sr[d] = (KD > 0 && d < D) ? LF(search, d) : 0.f;
     wave_sync();
     float g1 = 0.f, g2 = 0.f;
     for (int d = 0; d < D; d++) {
@@ -691,7 +704,7 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
     // order, from tile row q
     float acc = lane == 0 ? gauss : lane == 1 ? g1 : lane == 2 ? g2 : 0.f;
     // this lane's row of the next chunk is loaded while the current chunk is summed
-    float n_ja = 0.f, n_f = 0.f, n_r = 0.f, n_D = 0.f, n_j[KD];
+    float n_ja = 0.f, n_f = 0.f, n_r = 0.f, n_D = 0.f, n_j[KR];
     auto load = [&](int c) {
         if (c < nc) {
             n_ja = R.Jaref(c);
@@ -701,23 +714,27 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
                 n_r = R.diag(c);
             }
 #pragma unroll
-            for (int d = 0; d < KD; d++)
-                if (d < D) n_j[d] = R.jac(c, d);
+            for (int d = 0; d < KR; d++)
+                if (KD > 0 && d < D) n_j[d] = R.jac(c, d);
         }
     };
     load(lane);
     for (int c0 = 0; c0 < nc; c0 += kWave) {
         const int c = c0 + lane;
         const float ja = n_ja, Dc = n_D, fl_c = n_f, dg = n_r;
-        float jr[KD];
+        float jr[KR];
 #pragma unroll
-        for (int d = 0; d < KD; d++) jr[d] = n_j[d];
+        for (int d = 0; d < KR; d++) jr[d] = n_j[d];
         load(c + kWave);
         if (c < nc) {
             float jv = 0.f;
+            if constexpr (KD > 0) {
 #pragma unroll
-            for (int d = 0; d < KD; d++)
-                if (d < D) jv = jv + jr[d] * sr[d];
+                for (int d = 0; d < KD; d++)
+                    if (d < D) jv = jv + jr[d] * sr[d];
+            } else {
+                for (int d = 0; d < D; d++) jv = jv + R.jac(c, d) * LF(search, d);
+            }
             R.jv(c) = jv;
             float f0 = Dc * (0.5f * ja * ja), f1 = Dc * (jv * ja), f2 = Dc * (0.5f * jv * jv);
             if (c < ne) {
@@ -856,12 +873,10 @@ __device__ inline float wave_linesearch(const DModel& m, const LState& loc, cons
         return 0.f;
     }
     LsP p0;
-    if constexpr (KD > 0) p0 = wave_ls_init_r<KD>(m, loc, R, lane, ls_it, tile);
-    else p0 = wave_ls_init(m, loc, R, lane, ls_it);
+    p0 = wave_ls_init_r<KD>(m, loc, R, lane, ls_it, tile);
     LsP p1;
     float a = p0.alpha - p0.grad / p0.hess;
-    if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &a, &p1, ls_it, tile);
-    else wave_ls_eval(m, loc, R, lane, &a, 1, &p1, ls_it);
+    wave_ls_eval_r<1>(m, loc, R, lane, &a, &p1, ls_it, tile);
     if (p0.cost < p1.cost) p1 = p0;
     bool done = false;
     if (fabsf(p1.grad) < gtol) {
@@ -877,8 +892,7 @@ __device__ inline float wave_linesearch(const DModel& m, const LState& loc, cons
             p2 = p1;
             p2update = 1;
             float an = p1.alpha - p1.grad / p1.hess;
-            if constexpr (KD > 0) wave_ls_eval_r<1>(m, loc, R, lane, &an, &p1, ls_it, tile);
-            else wave_ls_eval(m, loc, R, lane, &an, 1, &p1, ls_it);
+            wave_ls_eval_r<1>(m, loc, R, lane, &an, &p1, ls_it, tile);
             if (fabsf(p1.grad) < gtol) {
                 res = p1.alpha;
                 done = true;
@@ -900,8 +914,7 @@ __device__ inline float wave_linesearch(const DModel& m, const LState& loc, cons
         float al[3] = {p1.alpha - p1.grad / p1.hess, p1.alpha, (p1.alpha + p2.alpha) * 0.5f};
         while (ls_it < m.ls_iterations) {
             LsP c[3];
-            if constexpr (KD > 0) wave_ls_eval_r<3>(m, loc, R, lane, al, c, ls_it, tile);
-            else wave_ls_eval(m, loc, R, lane, al, 3, c, ls_it);
+            wave_ls_eval_r<3>(m, loc, R, lane, al, c, ls_it, tile);
             float n1 = al[0], n2 = al[1];
             bool found = false;
             float ba = 0.f, bc = 0.f;
@@ -972,6 +985,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     if constexpr (KD > 0) wave_hessian_cholesky_tiled(m, loc, R, lane, tile);
     else wave_hessian_cholesky(m, loc, R, lane);
     if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+    else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
     else wave_gradient(m, loc);
     for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
     int n_iter = 0;
@@ -1005,6 +1019,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
                     else wave_hessian_cholesky(m, loc, R, lane);
                 }
                 if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+                else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
                 else wave_gradient(m, loc);
                 float gn = 0.f;
                 for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
