@@ -303,6 +303,11 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
 
     stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
     wave_sync();
+    // An environment that had many constraint rows in its last substep is likely the longest of this one: it takes
+    // issue priority over the other wavefronts of its SIMD, which shortens the step's tail (row-tile models, whose
+    // steps are bound by their few heavy environments; scheduling only, no result depends on it).
+    if constexpr (kPath == 1)
+        if (FP(loc, n_con)[0] >= kSchedHeavyRows) __builtin_amdgcn_s_setprio(3);
     // kernel_step_1: CRBA, L^T D L, actuation + passive + bias forces, acc_smooth
     wave_mass_matrix(m, loc, lane);
     wave_factor_mass(m, loc, lane);

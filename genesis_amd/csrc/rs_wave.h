@@ -49,8 +49,9 @@ __host__ __device__ constexpr int tile_rows(int n_dofs) {
 // substep reads, [1 + parity * kSchedClasses + class] the queue lengths.
 constexpr int kSchedClasses = 4;
 __host__ __device__ constexpr int sched_rows() { return 2 * kSchedClasses + 1; }
+constexpr int kSchedHeavyRows = 150;
 __device__ __forceinline__ int sched_class(int n_con) {
-    return n_con >= 150 ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
+    return n_con >= kSchedHeavyRows ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
 }
 
 // perturbed multi-contact detections of the fused narrowphase: per-pair record and per-detection result (floats)
