@@ -288,8 +288,8 @@ __device__ inline void wave_narrowphase(const DModel& m, const LState& loc, cons
 }
 
 // One environment's substep on the calling wavefront (w = wavefront index in the workgroup).
-// kPath: 1 = models with a row tile (at most kTileMaxDofs dofs), 0 = the others, 2 = either (decided per call: the
-// shared launch of several models).
+// kPath: 1 = models in the first row-tile class (at most kTileMaxDofs dofs), 4 = the second (up to kTileWideDofs),
+// 0 = the others, 2 = any (decided per call: the shared launch of several models).
 template <bool kLds, int kPath = 2>
 __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, const FusedArgs& fa, int b, int w,
                                           float* lds_env) {
@@ -306,7 +306,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     // An environment that had many constraint rows in its last substep is likely the longest of this one: it takes
     // issue priority over the other wavefronts of its SIMD, which shortens the step's tail (row-tile models, whose
     // steps are bound by their few heavy environments; scheduling only, no result depends on it).
-    if constexpr (kPath == 1)
+    if constexpr (kPath == 1 || kPath == 4)
         if (FP(loc, n_con)[0] >= kSchedHeavyRows) __builtin_amdgcn_s_setprio(3);
     // kernel_step_1: CRBA, L^T D L, actuation + passive + bias forces, acc_smooth
     wave_mass_matrix(m, loc, lane);
@@ -329,8 +329,10 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     if (!m.disable_constraint) {
         wave_constraints(m, loc, st, b, R, lane, fa.tiled ? tile : nullptr);
         if constexpr (kPath == 1) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, tile);
+        else if constexpr (kPath == 4) wave_solve<kTileWideDofs>(m, loc, st, b, R, lane, tile);
         else if constexpr (kPath == 0) wave_solve<0>(m, loc, st, b, R, lane, tile);
-        else if (fa.tiled) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, tile);
+        else if (fa.tiled == kTileMaxDofs) wave_solve<kTileMaxDofs>(m, loc, st, b, R, lane, tile);
+        else if (fa.tiled == kTileWideDofs) wave_solve<kTileWideDofs>(m, loc, st, b, R, lane, tile);
         else wave_solve<0>(m, loc, st, b, R, lane, tile);
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities

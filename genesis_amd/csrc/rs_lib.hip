@@ -131,7 +131,7 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->n_local = (int)off;
     off = up4(off);
     fa->loc_floats = (int)off;
-    fa->tiled = h.n_dofs <= kTileMaxDofs;
+    fa->tiled = tile_class(h.n_dofs);
     fa->tile_off = 0;  // set per launch by set_lds_slice
     fa->lds_wave = (int)up4((int64_t)kTileStride * tile_rows(h.n_dofs));
     fa->sched = 0;
@@ -333,7 +333,7 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     const size_t lds = (size_t)epb * fa.lds_wave * 4;
     // one kernel variant per (private region, occupancy) for models with a row tile; models without one run the
     // generic-solver variants at the compiler's register budget
-    if (fa.tiled) {
+    if (fa.tiled == kTileMaxDofs) {
         if (m->use_lds) {
             if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<true, 4, 1>), grid, block, lds, s, m->dm, st, fa);
             else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3, 1>), grid, block, lds, s, m->dm, st, fa);
@@ -342,6 +342,14 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
             if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<false, 4, 1>), grid, block, lds, s, m->dm, st, fa);
             else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3, 1>), grid, block, lds, s, m->dm, st, fa);
             else hipLaunchKernelGGL((k_fused_wave<false, 0, 1>), grid, block, lds, s, m->dm, st, fa);
+        }
+    } else if (fa.tiled == kTileWideDofs) {
+        if (m->use_lds) {
+            if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3, 4>), grid, block, lds, s, m->dm, st, fa);
+            else hipLaunchKernelGGL((k_fused_wave<true, 0, 4>), grid, block, lds, s, m->dm, st, fa);
+        } else {
+            if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3, 4>), grid, block, lds, s, m->dm, st, fa);
+            else hipLaunchKernelGGL((k_fused_wave<false, 0, 4>), grid, block, lds, s, m->dm, st, fa);
         }
     } else {
         if (m->use_lds) hipLaunchKernelGGL((k_fused_wave<true, 0, 0>), grid, block, lds, s, m->dm, st, fa);
@@ -478,6 +486,8 @@ int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     if (int r = check_fused_config(m, c->envs_per_block, c->use_lds != 0)) return r;
     if (!m->fa.tiled && c->occupancy != 0)
         return fail(RS_E_UNSUPPORTED, "fused config: occupancy variants exist for models with a row tile only");
+    if (m->fa.tiled == kTileWideDofs && c->occupancy == 4)
+        return fail(RS_E_UNSUPPORTED, "fused config: models with 17 to 24 dofs have occupancy variants 0 and 3 only");
     m->fa.epb = c->envs_per_block;
     m->use_lds = c->use_lds != 0;
     m->schedule = c->schedule != 0;

@@ -22,7 +22,7 @@ struct FusedArgs {
     int epb;           // environments (wavefronts) per workgroup
     int n_local;       // local rows staged per environment (the packed local planes)
     int loc_floats;    // private region per environment (n_local rounded up to 16 bytes)
-    int tiled;         // 1: the model has at most kTileMaxDofs dofs (the row-tile solver variants apply)
+    int tiled;         // row-tile solver class: kTileMaxDofs or kTileWideDofs (the KD its variants are built for), 0: none
     int lds_wave;      // floats of LDS per wavefront: [private region if in LDS][row tile]
     int64_t work_stride;  // floats of the work field per environment
     int off_loc, off_jac, off_rowf, off_tmp, off_pc, off_mc;
@@ -36,10 +36,14 @@ struct FusedArgs {
 // that a full row is read with 16-byte loads, and puts rows q and q + 1 four banks apart, so that up to 16 lanes
 // reading different rows at the same column, and 64 lanes writing one row, hit distinct banks.
 constexpr int kTileStride = 68;
-constexpr int kTileMaxDofs = 16;
-constexpr int kTileMinRows = 12;  // the ordered sums of the line search use up to 9 rows
+constexpr int kTileMaxDofs = 16;   // row-tile solver variants for models with up to 16 dofs...
+constexpr int kTileWideDofs = 24;  // ... and a second class for 17 to 24 dofs (wider register rows)
+constexpr int kTileMinRows = 12;   // the ordered sums of the line search use up to 9 rows
 __host__ __device__ constexpr int tile_rows(int n_dofs) {
-    return n_dofs <= kTileMaxDofs && n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;
+    return n_dofs <= kTileWideDofs && n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;
+}
+__host__ __device__ constexpr int tile_class(int n_dofs) {
+    return n_dofs <= kTileMaxDofs ? kTileMaxDofs : n_dofs <= kTileWideDofs ? kTileWideDofs : 0;
 }
 
 // Schedule: after its substep each environment appends itself to one of kSchedClasses queues by cost (its number of

@@ -354,7 +354,7 @@ __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, i
 // in registers and stages them in the tile, and every sum over rows runs in one lane per output, in row order
 // (tile_seq_sums), so many ordered sums proceed side by side. The line-search passes serve every model (KD = 0:
 // any number of dofs); the constraint update, Hessian, rank-1 update and triangular-solve variants need at most KD
-// dofs (instantiated for KD = 16). Same arithmetic and order as the functions above.
+// dofs (instantiated for KD = 16 and KD = 24). Same arithmetic and order as the functions above.
 
 // solver.py:2620-2694
 template <int KD>
@@ -450,13 +450,14 @@ __device__ inline float wave_update_constraint_r(const DModel& m, const LState& 
 
 // solver.py:1395-1452 + :1595-1621 with each chunk of rows staged in the row tile: one lane per Hessian entry sums
 // its terms over the rows in row order from LDS
+template <int KD>
 __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState& loc, const WorkRows& R, int lane,
                                                    float* tile) {
     const int D = m.D;
     const int nc = LF(n_con, 0);
     const int nE = D * (D + 1) / 2;
     const float eps = m.eps;
-    constexpr int kE = (kTileMaxDofs * (kTileMaxDofs + 1) / 2 + kWave - 1) / kWave;  // entries per lane
+    constexpr int kE = (KD * (KD + 1) / 2 + kWave - 1) / kWave;  // entries per lane
     float h[kE];
     int ei[kE], ej[kE];
 #pragma unroll
@@ -470,14 +471,14 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState
         ej[r] = e - i * (i + 1) / 2;
     }
     // this lane's row of the next chunk is loaded while the current chunk is summed
-    float n_j[kTileMaxDofs], n_D = 0.f;
+    float n_j[KD], n_D = 0.f;
     int n_act = 0;
     auto load = [&](int c) {
         if (c < nc) {
             n_D = R.efc_D(c);
             n_act = R.active(c);
 #pragma unroll
-            for (int d = 0; d < kTileMaxDofs; d++)
+            for (int d = 0; d < KD; d++)
                 if (d < D) n_j[d] = R.jac(c, d);
         }
     };
@@ -486,7 +487,7 @@ __device__ inline void wave_hessian_cholesky_tiled(const DModel& m, const LState
         const int c = c0 + lane;
         if (c < nc) {
 #pragma unroll
-            for (int d = 0; d < kTileMaxDofs; d++)
+            for (int d = 0; d < KD; d++)
                 if (d < D) tile[d * kTileStride + lane] = n_j[d];
             tile[D * kTileStride + lane] = n_D;
             tile[(D + 1) * kTileStride + lane] = (float)n_act;
@@ -982,7 +983,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     else cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
     LF(cost, 0) = cost;
     int improved = nc > 0;
-    if constexpr (KD > 0) wave_hessian_cholesky_tiled(m, loc, R, lane, tile);
+    if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
     else wave_hessian_cholesky(m, loc, R, lane);
     if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
     else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
@@ -1015,7 +1016,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
                     degenerate = wave_cholesky_incremental(m, loc, R, lane);
                 }
                 if (degenerate) {
-                    if constexpr (KD > 0) wave_hessian_cholesky_tiled(m, loc, R, lane, tile);
+                    if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
                     else wave_hessian_cholesky(m, loc, R, lane);
                 }
                 if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);

@@ -185,6 +185,35 @@ def test_fused_quadruped_equals_plain_and_tracks_oracle():
         assert err < 5e-5, f"{name}: {err:.3e}"
 
 
+@pytest.mark.parametrize("shape", [(2, True, False, 0), (1, False, False, 0), (2, True, False, 3), (4, False, True, 3)])
+def test_fused_quadruped_launch_shapes_equal_plain(shape):
+    """18 dofs: the second row-tile class (17 to 24 dofs), in each private-region placement and occupancy variant,
+    over 5 substeps in contact (rank-1 updates and line searches on every row count the gait produces)."""
+    rt = _rt()
+    from genesis_amd.scenes import quadruped_ctrl_mode, quadruped_initial_qpos, quadruped_model, quadruped_targets
+
+    B = 256
+    model = quadruped_model()
+    dm = rt.DeviceModel(model)
+    dm.set_fused_config(*shape)
+    a = rt.StateSlab(dm, B)
+    a["qpos"].copy_(torch.as_tensor(quadruped_initial_qpos(model, B).T, dtype=torch.float32))
+    a["ctrl_mode"].copy_(torch.as_tensor(quadruped_ctrl_mode(model, B).T, dtype=torch.int32))
+    a["ctrl_pos"].copy_(torch.as_tensor(quadruped_targets(model, B, 0).T, dtype=torch.float32))
+    a.forward_kinematics()
+    a.step(80, rt.RS_MODE_PLAIN)
+    torch.cuda.synchronize()
+    b = _twin(rt, dm, a)
+    print(f"[fused] quadruped B={B} launch shape {shape} config={dm.fused_config}", flush=True)
+    a.step(5, rt.RS_MODE_PLAIN)
+    b.step(5, rt.RS_MODE_FUSED)
+    torch.cuda.synchronize()
+    assert int(get_dev(a, "n_con").max()) > 0
+    for name in OUTPUTS:
+        d = _max_abs(get_dev(a, name), get_dev(b, name))
+        assert d == 0.0, f"{shape} {name}: fused differs from per-stage by {d:.3e}"
+
+
 def test_autotune_keeps_state_and_picks_a_shape():
     rt = _rt()
     model, dm, slab = _franka(512, 10)
