@@ -92,11 +92,11 @@ struct RowJac {
     __device__ __forceinline__ void flush(const R&, int, int) const {}
 };
 
-// solver.py:580-685 add_collision_constraints for contact ic: rows start + 4 ic .. + 3. The impedance depends on the
-// contact only and is evaluated once for its four rows.
+// solver.py:580-685 add_collision_constraints, row i (0..3) of contact ic: row start + 4 ic + i (pyramid direction
+// +-d1, +-d2 from the contact normal). The impedance depends on the contact only.
 template <class R, class S, class J = RowJac>
-__device__ inline void contact_rows(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
-                                    int ic, int start, const J& jacc = J()) {
+__device__ inline void contact_row(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
+                                   int ic, int i, int start, const J& jacc = J()) {
     const int D = m.D;
     const size_t CB = (size_t)cs.B;
     int la = cs.contact_link_a[ic * CB + cb], lb = cs.contact_link_b[ic * CB + cb];
@@ -109,36 +109,41 @@ __device__ inline void contact_rows(const DModel& m, const S& st, int b, const D
     float invw = m.link_invweight[2 * la];
     if (lb > -1) invw = invw + m.link_invweight[2 * lb];
     const float imp = impedance(sp, -pen);
-    for (int i = 0; i < 4; i++) {
-        f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
-        f3 n = dd * fr - nrm;
-        int c = start + ic * 4 + i;
-        for (int k = 0; k < D; k++) jacc.at(rows, c, k) = 0.f;
-        float jq = 0.f;
-        for (int ab = 0; ab < 2; ab++) {
-            float sign = ab == 0 ? -1.f : 1.f;
-            int link = ab == 0 ? la : lb;
-            while (link > -1) {
-                f3 tpos = cpos - LD3(root_COM, link);
-                for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
-                    f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
-                    f3 vel = qrot(cv - cross3(tpos, ca), qident());
-                    float jj = dot3(sign * vel, n);
-                    jq = jq + jj * SF(vel, k);
-                    jacc.at(rows, c, k) = jacc.at(rows, c, k) + jj;
-                }
-                link = m.link_parent[link];
+    f3 dd = (float)(2 * (i % 2) - 1) * (i < 2 ? d1 : d2);
+    f3 n = dd * fr - nrm;
+    int c = start + ic * 4 + i;
+    for (int k = 0; k < D; k++) jacc.at(rows, c, k) = 0.f;
+    float jq = 0.f;
+    for (int ab = 0; ab < 2; ab++) {
+        float sign = ab == 0 ? -1.f : 1.f;
+        int link = ab == 0 ? la : lb;
+        while (link > -1) {
+            f3 tpos = cpos - LD3(root_COM, link);
+            for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
+                f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
+                f3 vel = qrot(cv - cross3(tpos, ca), qident());
+                float jj = dot3(sign * vel, n);
+                jq = jq + jj * SF(vel, k);
+                jacc.at(rows, c, k) = jacc.at(rows, c, k) + jj;
             }
+            link = m.link_parent[link];
         }
-        jacc.flush(rows, c, D);
-        const float aref = reference_acc(sp, imp, jq, -pen);
-        float dg = invw + fr * fr * invw;
-        dg *= 2.f * fr * fr * (1.f - imp) / imp;
-        dg = fmaxf(dg, m.eps);
-        rows.diag(c) = dg;
-        rows.aref(c) = aref;
-        rows.efc_D(c) = 1.f / dg;
     }
+    jacc.flush(rows, c, D);
+    const float aref = reference_acc(sp, imp, jq, -pen);
+    float dg = invw + fr * fr * invw;
+    dg *= 2.f * fr * fr * (1.f - imp) / imp;
+    dg = fmaxf(dg, m.eps);
+    rows.diag(c) = dg;
+    rows.aref(c) = aref;
+    rows.efc_D(c) = 1.f / dg;
+}
+
+// the four rows of contact ic: rows start + 4 ic .. + 3
+template <class R, class S, class J = RowJac>
+__device__ inline void contact_rows(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
+                                    int ic, int start, const J& jacc = J()) {
+    for (int i = 0; i < 4; i++) contact_row(m, st, b, cs, cb, rows, ic, i, start, jacc);
 }
 
 // solver.py:1177-1233 add_joint_limit_constraints; appends from row nc, returns the new row count

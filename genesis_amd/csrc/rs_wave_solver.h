@@ -40,7 +40,8 @@ struct TileJac {
     }
 };
 
-// solver.py:580-1286: equality, friction-loss and limit rows uniformly (few rows), contact rows one lane per contact
+// solver.py:580-1286: equality, friction-loss and limit rows uniformly (few rows), contact rows one lane per row (with a
+// row tile) or per contact
 __device__ inline void wave_constraints(const DModel& m, const LState& loc, const DState& st, int b, const WorkRows& R,
                                         int lane, float* tile) {
     int nc = equality_rows(m, loc, 0, R);
@@ -49,9 +50,9 @@ __device__ inline void wave_constraints(const DModel& m, const LState& loc, cons
     nc += nfl;
     if (m.enable_collision) {
         const int ncont = LF(n_contacts, 0);
-        if (tile) {
+        if (tile) {  // one lane per row: lane 4 q + i builds row i of the q-th contact of the pass
             const TileJac jt{tile + lane};
-            for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc, jt);
+            for (int r = lane; r < 4 * ncont; r += kWave) contact_row(m, loc, 0, st, b, R, r >> 2, r & 3, nc, jt);
         } else {
             for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc);
         }
