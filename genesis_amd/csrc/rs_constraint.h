@@ -92,11 +92,28 @@ struct RowJac {
     __device__ __forceinline__ void flush(const R&, int, int) const {}
 };
 
+// The kinematic tree as the jacobian walk reads it (parent link, dof range of a link): from the model (ChainModel), or
+// from a copy that the fused kernel stages in LDS (ChainLds: parent, dof start, dof end, L entries each), since the
+// walk's reads at per-lane addresses are a chain of dependent loads.
+struct ChainModel {
+    const DModel& m;
+    __device__ __forceinline__ int parent(int l) const { return m.link_parent[l]; }
+    __device__ __forceinline__ int dof_start(int l) const { return m.link_dof_start[l]; }
+    __device__ __forceinline__ int dof_end(int l) const { return m.link_dof_end[l]; }
+};
+struct ChainLds {
+    const int* p;
+    int L;
+    __device__ __forceinline__ int parent(int l) const { return p[l]; }
+    __device__ __forceinline__ int dof_start(int l) const { return p[L + l]; }
+    __device__ __forceinline__ int dof_end(int l) const { return p[2 * L + l]; }
+};
+
 // solver.py:580-685 add_collision_constraints, row i (0..3) of contact ic: row start + 4 ic + i (pyramid direction
 // +-d1, +-d2 from the contact normal). The impedance depends on the contact only.
-template <class R, class S, class J = RowJac>
+template <class R, class S, class J, class C>
 __device__ inline void contact_row(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
-                                   int ic, int i, int start, const J& jacc = J()) {
+                                   int ic, int i, int start, const J& jacc, const C& chain) {
     const int D = m.D;
     const size_t CB = (size_t)cs.B;
     int la = cs.contact_link_a[ic * CB + cb], lb = cs.contact_link_b[ic * CB + cb];
@@ -119,14 +136,15 @@ __device__ inline void contact_row(const DModel& m, const S& st, int b, const DS
         int link = ab == 0 ? la : lb;
         while (link > -1) {
             f3 tpos = cpos - LD3(root_COM, link);
-            for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
+            const int k0 = chain.dof_start(link), k1 = chain.dof_end(link), lp = chain.parent(link);
+            for (int k = k1 - 1; k >= k0; k--) {
                 f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
                 f3 vel = qrot(cv - cross3(tpos, ca), qident());
                 float jj = dot3(sign * vel, n);
                 jq = jq + jj * SF(vel, k);
                 jacc.at(rows, c, k) = jacc.at(rows, c, k) + jj;
             }
-            link = m.link_parent[link];
+            link = lp;
         }
     }
     jacc.flush(rows, c, D);
@@ -137,6 +155,12 @@ __device__ inline void contact_row(const DModel& m, const S& st, int b, const DS
     rows.diag(c) = dg;
     rows.aref(c) = aref;
     rows.efc_D(c) = 1.f / dg;
+}
+
+template <class R, class S, class J = RowJac>
+__device__ inline void contact_row(const DModel& m, const S& st, int b, const DState& cs, int cb, const R& rows,
+                                   int ic, int i, int start, const J& jacc = J()) {
+    contact_row(m, st, b, cs, cb, rows, ic, i, start, jacc, ChainModel{m});
 }
 
 // the four rows of contact ic: rows start + 4 ic .. + 3

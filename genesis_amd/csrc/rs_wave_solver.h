@@ -52,7 +52,22 @@ __device__ inline void wave_constraints(const DModel& m, const LState& loc, cons
         const int ncont = LF(n_contacts, 0);
         if (tile) {  // one lane per row: lane 4 q + i builds row i of the q-th contact of the pass
             const TileJac jt{tile + lane};
-            for (int r = lane; r < 4 * ncont; r += kWave) contact_row(m, loc, 0, st, b, R, r >> 2, r & 3, nc, jt);
+            // rows D .. D + 2 of the tile are free until the solve: the walk's tree arrays go there when they fit
+            const int D = m.D, L = m.L;
+            if (3 * L <= 3 * kTileStride && tile_rows(D) >= D + 3) {
+                int* tree = (int*)(tile + D * kTileStride);
+                for (int l = lane; l < L; l += kWave) {
+                    tree[l] = m.link_parent[l];
+                    tree[L + l] = m.link_dof_start[l];
+                    tree[2 * L + l] = m.link_dof_end[l];
+                }
+                wave_sync();
+                const ChainLds ch{tree, L};
+                for (int r = lane; r < 4 * ncont; r += kWave)
+                    contact_row(m, loc, 0, st, b, R, r >> 2, r & 3, nc, jt, ch);
+            } else {
+                for (int r = lane; r < 4 * ncont; r += kWave) contact_row(m, loc, 0, st, b, R, r >> 2, r & 3, nc, jt);
+            }
         } else {
             for (int ic = lane; ic < ncont; ic += kWave) contact_rows(m, loc, 0, st, b, R, ic, nc);
         }
