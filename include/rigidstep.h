@@ -171,8 +171,9 @@ int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_
  * envs_per_block wavefronts per workgroup (1..8), whether the environment's small fields are staged in LDS
  * (use_lds = 1, only when they fit) or in the env-major work field, and whether environments are started in order of
  * their previous substep's cost (schedule = 1; every environment's result is the same in any order), and the kernel's
- * register budget (occupancy 0, 3 or 4). rs_fused_config_set rejects configurations the
- * device cannot run. The model keeps the configuration for every later RS_MODE_FUSED step. */
+ * register budget (occupancy 0, 3 or 4; models with at most 16 dofs have all three, models with 17 to 24 dofs 0 and 3,
+ * larger models 0 only). rs_fused_config_set rejects configurations the device cannot run. The model keeps the
+ * configuration for every later RS_MODE_FUSED step. */
 typedef struct rs_fused_config {
     int32_t envs_per_block;
     int32_t use_lds;
