@@ -370,13 +370,18 @@ __device__ inline void wave_update_geoms(const DModel& m, const LState& loc, int
     }
 }
 
-// forward_kinematics.py:580-736 with the per-joint work that does not depend on the chain done first, one lane per
-// joint: the joint's local rotation (revolute: from its angle; spherical and free: from its coordinates) and its
-// dof positions. The chain walk then runs on every lane with the same arithmetic as dev_forward_kinematics, taking a
-// joint's precomputed values from its lane and keeping the anchor, axis and the previous link's pose in registers
-// rather than reading back what it has just stored. Models with more than 64 joints take dev_forward_kinematics.
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ f3 readlane3(f3 v, int l) { return mk3(readlane_f(v.x, l), readlane_f(v.y, l), readlane_f(v.z, l)); }
+
+// forward_kinematics.py:580-736 with the work that does not depend on the chain done first, one lane per link and
+// per joint: the links' and joints' constants (offsets, parents, joint ranges, anchors, axes), the joint's local
+// rotation (revolute: from its angle; spherical and free: from its coordinates) and its dof positions. The chain walk
+// then runs on every lane with the same arithmetic as dev_forward_kinematics, taking a link's or joint's values from
+// its lane (the model reads would otherwise be a chain of dependent loads per link) and keeping the anchor, axis and
+// the previous link's pose in registers rather than reading back what it has just stored. Models with more than 64
+// links or joints take dev_forward_kinematics.
 __device__ inline void wave_forward_kinematics(const DModel& m, const LState& loc, int lane) {
-    if (m.J > kWave) {
+    if (m.J > kWave || m.L > kWave) {
         dev_forward_kinematics(m, loc, 0);
         return;
     }
@@ -384,9 +389,16 @@ __device__ inline void wave_forward_kinematics(const DModel& m, const LState& lo
     const int b = 0;
     f4q jq{1.f, 0.f, 0.f, 0.f};  // lane j: local rotation of joint j (free: the normalised orientation)
     float jd = 0.f;               // lane j: revolute / prismatic displacement
+    int jtype = JT_FIXED, jqs = 0;
+    f3 jpos = zero3(), jaxis = mk3(0.f, 0.f, 1.f);
     if (lane < m.J) {
         const int j = lane;
         const int jt = m.joint_type[j], qs = m.joint_q_start[j], ds = m.joint_dof_start[j];
+        jtype = jt;
+        jqs = qs;
+        jpos = mvec3(m.joint_pos, j);
+        if (jt == JT_REVOLUTE) jaxis = mvec3(m.dof_motion_ang, ds);
+        else if (jt == JT_PRISMATIC) jaxis = mvec3(m.dof_motion_vel, ds);
         if (jt == JT_FREE) {
             f4q q{SF(qpos, qs + 3), SF(qpos, qs + 4), SF(qpos, qs + 5), SF(qpos, qs + 6)};
             float n = sqrtf(nsq4(q));
@@ -409,33 +421,44 @@ __device__ inline void wave_forward_kinematics(const DModel& m, const LState& lo
             SF(dof_pos, ds) = jd;
         }
     }
+    f3 lpos = zero3();
+    f4q lquat{1.f, 0.f, 0.f, 0.f};
+    int lpar = -1, ljs = 0, lje = 0, lfix = 0;
+    if (lane < m.L) {
+        lpos = mvec3(m.link_pos, lane);
+        lquat = mquat(m.link_quat, lane);
+        lpar = m.link_parent[lane];
+        ljs = m.link_joint_start[lane];
+        lje = m.link_joint_end[lane];
+        lfix = m.link_is_fixed[lane];
+    }
     f3 prev_pos = zero3();
     f4q prev_quat{1.f, 0.f, 0.f, 0.f};
     int prev = -1;  // link whose pose prev_pos / prev_quat hold (stored in this walk)
     for (int l = 0; l < m.L; l++) {
-        f3 pos = mvec3(m.link_pos, l);
-        f4q quat = mquat(m.link_quat, l);
-        const int par = m.link_parent[l];
+        const f3 lp = readlane3(lpos, l);
+        const f4q lq{readlane_f(lquat.w, l), readlane_f(lquat.x, l), readlane_f(lquat.y, l), readlane_f(lquat.z, l)};
+        const int par = readlane_i(lpar, l), j0 = readlane_i(ljs, l), j1 = readlane_i(lje, l);
+        f3 pos = lp;
+        f4q quat = lq;
         if (par != -1) {
             const f3 pp = par == prev ? prev_pos : LD3(links_pos, par);
             const f4q pq = par == prev ? prev_quat : LD4(links_quat, par);
-            pos = pp + qrot(mvec3(m.link_pos, l), pq);
-            quat = qtrans(mquat(m.link_quat, l), pq);
+            pos = pp + qrot(lp, pq);
+            quat = qtrans(lq, pq);
         }
-        for (int j = m.link_joint_start[l]; j < m.link_joint_end[l]; j++) {
-            const int jt = m.joint_type[j], qs = m.joint_q_start[j], ds = m.joint_dof_start[j];
+        for (int j = j0; j < j1; j++) {
+            const int jt = readlane_i(jtype, j), qs = readlane_i(jqs, j);
             const f4q ql{readlane_f(jq.w, j), readlane_f(jq.x, j), readlane_f(jq.y, j), readlane_f(jq.z, j)};
             const float dp = readlane_f(jd, j);
+            const f3 jp = readlane3(jpos, j);
             f3 xanchor = zero3(), xaxis = zero3();
             if (jt == JT_FREE) {
                 xanchor = mk3(SF(qpos, qs), SF(qpos, qs + 1), SF(qpos, qs + 2));
                 xaxis = mk3(0.f, 0.f, 1.f);
             } else if (jt != JT_FIXED) {
-                f3 axis = mk3(0.f, 0.f, 1.f);
-                if (jt == JT_REVOLUTE) axis = mvec3(m.dof_motion_ang, ds);
-                else if (jt == JT_PRISMATIC) axis = mvec3(m.dof_motion_vel, ds);
-                xanchor = qrot(mvec3(m.joint_pos, j), quat) + pos;
-                xaxis = qrot(axis, quat);
+                xanchor = qrot(jp, quat) + pos;
+                xaxis = qrot(readlane3(jaxis, j), quat);
             }
             if (jt != JT_FIXED) {
                 ST3(xanchor, j, xanchor);
@@ -446,15 +469,15 @@ __device__ inline void wave_forward_kinematics(const DModel& m, const LState& lo
                 quat = ql;
             } else if (jt == JT_SPHERICAL) {
                 quat = qtrans(ql, quat);
-                pos = xanchor - qrot(mvec3(m.joint_pos, j), quat);
+                pos = xanchor - qrot(jp, quat);
             } else if (jt == JT_REVOLUTE) {
                 quat = qtrans(ql, quat);
-                pos = xanchor - qrot(mvec3(m.joint_pos, j), quat);
+                pos = xanchor - qrot(jp, quat);
             } else if (jt == JT_PRISMATIC) {
                 pos = pos + xaxis * dp;
             }
         }
-        if (!(par == -1 && m.link_is_fixed[l])) {
+        if (!(par == -1 && readlane_i(lfix, l))) {
             ST3(links_pos, l, pos);
             ST4(links_quat, l, quat);
             prev = l;
