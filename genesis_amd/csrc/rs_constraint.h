@@ -21,38 +21,168 @@ struct PlaneRows {
 
 // Rows are built from the kinematic state (st, b); contact records are read from (cs, cb).
 
-// solver.py:801-878 func_equality_joint via :881-938; returns the number of rows written from row 0
-template <class R, class S>
-__device__ inline int equality_rows(const DModel& m, const S& st, int b, const R& rows) {
-    const int D = m.D;
-    int nc = 0;
-    for (int e = 0; e < m.NE; e++) {
-        if (m.eq_type[e] != EQ_JOINT) continue;
-        cfptr sp = &m.eq_sol_params[7 * e];
-        cfptr data = &m.eq_data[11 * e];
-        int j1 = m.eq_obj1[e], j2 = m.eq_obj2[e];
-        int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
-        int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
-        int c = nc++;
-        for (int d = 0; d < D; d++) rows.jac(c, d) = 0.f;
-        float diff = SF(qpos, q2) - m.qpos0[q2];
-        float pos = SF(qpos, q1) - m.qpos0[q1];
-        float deriv = 0.f, dp = 1.f;
-        for (int i5 = 0; i5 < 5; i5++) {
-            pos = pos - dp * data[i5];
-            if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * (float)(i5 + 1);
-            dp = dp * diff;
+// jacobian row of a point p attached to `link`, component i of sign * (cdof_vel - (p - root_COM) x cdof_ang), added into
+// row c along the link's chain (dofs of each link last to first); returns the row's contribution to J v. kRot: the
+// velocity goes through the identity-rotation transform of solver.py:757-760 (connect rows), else it is taken as is
+// (weld rows, :1086-1089).
+template <bool kRot, class R, class S>
+__device__ inline float point_jac_row(const DModel& m, const S& st, int b, const R& rows, int c, int link, f3 p,
+                                      float sign, int i) {
+    float jq = 0.f;
+    while (link > -1) {
+        for (int k = m.link_dof_end[link] - 1; k >= m.link_dof_start[link]; k--) {
+            f3 ca = LD3(cdof_ang, k), cv = LD3(cdof_vel, k);
+            f3 tpos = p - LD3(root_COM, link);
+            f3 vel = cv - cross3(tpos, ca);
+            if (kRot) vel = qrot(vel, qident());
+            const float j = (sign * vel)[i];
+            jq = jq + j * SF(vel, k);
+            rows.jac(c, k) = rows.jac(c, k) + j;
         }
-        rows.jac(c, d1) = 1.f;
-        rows.jac(c, d2) = -deriv;
-        float jq = rows.jac(c, d1) * SF(vel, d1) + rows.jac(c, d2) * SF(vel, d2);
-        float invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+        link = m.link_parent[link];
+    }
+    return jq;
+}
+
+// solver.py:689-797 func_equality_connect: three rows holding the two anchors together
+template <class R, class S>
+__device__ inline void connect_rows(const DModel& m, const S& st, int b, const R& rows, int e, int& nc) {
+    const int D = m.D;
+    cfptr sp = &m.eq_sol_params[7 * e];
+    cfptr data = &m.eq_data[11 * e];
+    const int l1 = m.eq_obj1[e], l2 = m.eq_obj2[e];
+    const f3 g1 = tr_point(mk3(data[0], data[1], data[2]), LD3(links_pos, l1), LD4(links_quat, l1));
+    const f3 g2 = tr_point(mk3(data[3], data[4], data[5]), LD3(links_pos, l2), LD4(links_quat, l2));
+    const float invw = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+    for (int i3 = 0; i3 < 3; i3++) {
+        const int c = nc++;
+        for (int d = 0; d < D; d++) rows.jac(c, d) = 0.f;
+        float jq = 0.f;
+        jq = jq + point_jac_row<true>(m, st, b, rows, c, l1, g1, 1.f, i3);
+        jq = jq + point_jac_row<true>(m, st, b, rows, c, l2, g2, -1.f, i3);
+        const f3 pd = g1 - g2;
+        const float pen = norm3(pd);
         float imp, aref;
-        imp_aref(sp, -fabsf(pos), jq, pos, imp, aref);
-        float dg = fmaxf(invw * (1.f - imp) / imp, m.eps);
+        imp_aref(sp, -pen, jq, pd[i3], imp, aref);
+        const float dg = fmaxf(invw * (1.f - imp) / imp, m.eps);
         rows.diag(c) = dg;
         rows.aref(c) = aref;
         rows.efc_D(c) = 1.f / dg;
+    }
+}
+
+// solver.py:984-1173 func_equality_weld: three position rows, then three orientation rows
+template <class R, class S>
+__device__ inline void weld_rows(const DModel& m, const S& st, int b, const R& rows, int e, int& nc) {
+    const int D = m.D;
+    cfptr sp = &m.eq_sol_params[7 * e];
+    cfptr data = &m.eq_data[11 * e];
+    const int l1 = m.eq_obj1[e], l2 = m.eq_obj2[e];
+    const f4q relpose{data[6], data[7], data[8], data[9]};
+    const float ts = data[10];
+    const f4q q1 = LD4(links_quat, l1), q2 = LD4(links_quat, l2);
+    const f3 g1 = tr_point(mk3(data[3], data[4], data[5]), LD3(links_pos, l1), q1);
+    const f3 g2 = tr_point(mk3(data[0], data[1], data[2]), LD3(links_pos, l2), q2);
+    const f3 pe = g1 - g2;
+    const f4q q = qmul(q1, relpose);
+    const f4q iq2 = qinv(q2);
+    const f4q eq = qmul(iq2, q);
+    const f3 re = mk3(eq.x, eq.y, eq.z) * ts;
+    const float all[6] = {pe.x, pe.y, pe.z, re.x, re.y, re.z};
+    float sq = 0.f;
+    for (int k = 0; k < 6; k++) sq = sq + all[k] * all[k];
+    const float pos_imp = sqrtf(sq);
+    const float invw0 = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+    const float invw1 = m.link_invweight[2 * l1 + 1] + m.link_invweight[2 * l2 + 1];
+    for (int i = 0; i < 3; i++) {
+        const int c = nc++;
+        for (int d = 0; d < D; d++) rows.jac(c, d) = 0.f;
+        float jq = 0.f;
+        jq = jq + point_jac_row<false>(m, st, b, rows, c, l1, g1, 1.f, i);
+        jq = jq + point_jac_row<false>(m, st, b, rows, c, l2, g2, -1.f, i);
+        float imp, aref;
+        imp_aref(sp, -pos_imp, jq, pe[i], imp, aref);
+        const float dg = fmaxf(invw0 * (1.f - imp) / imp, m.eps);
+        rows.diag(c) = dg;
+        rows.aref(c) = aref;
+        rows.efc_D(c) = 1.f / dg;
+    }
+    const int c0 = nc;
+    nc += 3;
+    for (int k = 0; k < 3; k++)
+        for (int d = 0; d < D; d++) rows.jac(c0 + k, d) = 0.f;
+    for (int ab = 0; ab < 2; ab++) {
+        const float sign = ab == 0 ? 1.f : -1.f;
+        int link = ab == 0 ? l1 : l2;
+        while (link > -1) {
+            for (int d = m.link_dof_end[link] - 1; d >= m.link_dof_start[link]; d--) {
+                const f3 jv = sign * LD3(cdof_ang, d);
+                for (int k = 0; k < 3; k++) rows.jac(c0 + k, d) = rows.jac(c0 + k, d) + jv[k];
+            }
+            link = m.link_parent[link];
+        }
+    }
+    float jq3[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; d++) {
+        const f3 jd = mk3(rows.jac(c0, d), rows.jac(c0 + 1, d), rows.jac(c0 + 2, d));
+        const f4q quat3 = qmul(qmul_axis(iq2, jd), q);
+        const float qv[3] = {quat3.x, quat3.y, quat3.z};
+        for (int k = 0; k < 3; k++) {
+            rows.jac(c0 + k, d) = 0.5f * qv[k] * ts;
+            jq3[k] = jq3[k] + rows.jac(c0 + k, d) * SF(vel, d);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        float imp, aref;
+        imp_aref(sp, -pos_imp, jq3[k], re[k], imp, aref);
+        const float dg = fmaxf(invw1 * (1.f - imp) / imp, m.eps);
+        rows.diag(c0 + k) = dg;
+        rows.aref(c0 + k) = aref;
+        rows.efc_D(c0 + k) = 1.f / dg;
+    }
+}
+
+// solver.py:801-878 func_equality_joint
+template <class R, class S>
+__device__ inline void joint_equality_row(const DModel& m, const S& st, int b, const R& rows, int e, int& nc) {
+    const int D = m.D;
+    cfptr sp = &m.eq_sol_params[7 * e];
+    cfptr data = &m.eq_data[11 * e];
+    int j1 = m.eq_obj1[e], j2 = m.eq_obj2[e];
+    int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
+    int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
+    int c = nc++;
+    for (int d = 0; d < D; d++) rows.jac(c, d) = 0.f;
+    float diff = SF(qpos, q2) - m.qpos0[q2];
+    float pos = SF(qpos, q1) - m.qpos0[q1];
+    float deriv = 0.f, dp = 1.f;
+    for (int i5 = 0; i5 < 5; i5++) {
+        pos = pos - dp * data[i5];
+        if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * (float)(i5 + 1);
+        dp = dp * diff;
+    }
+    rows.jac(c, d1) = 1.f;
+    rows.jac(c, d2) = -deriv;
+    float jq = rows.jac(c, d1) * SF(vel, d1) + rows.jac(c, d2) * SF(vel, d2);
+    float invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+    float imp, aref;
+    imp_aref(sp, -fabsf(pos), jq, pos, imp, aref);
+    float dg = fmaxf(invw * (1.f - imp) / imp, m.eps);
+    rows.diag(c) = dg;
+    rows.aref(c) = aref;
+    rows.efc_D(c) = 1.f / dg;
+}
+
+// solver.py:881-938 add_equality_constraints: the rows of every equality in equality order (connect 3, weld 6,
+// joint 1); returns the number of rows
+template <class R, class S>
+__device__ inline int equality_rows(const DModel& m, const S& st, int b, const R& rows) {
+    int nc = 0;
+    for (int e = 0; e < m.NE; e++) {
+        const int t = m.eq_type[e];
+        if (t == EQ_CONNECT) connect_rows(m, st, b, rows, e, nc);
+        else if (t == EQ_WELD) weld_rows(m, st, b, rows, e, nc);
+        else if (t == EQ_JOINT) joint_equality_row(m, st, b, rows, e, nc);
     }
     return nc;
 }

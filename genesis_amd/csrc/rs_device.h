@@ -149,6 +149,11 @@ __device__ __forceinline__ f4q qmul(f4q u, f4q v) {
     return {u.w * v.w - u.x * v.x - u.y * v.y - u.z * v.z, u.w * v.x + u.x * v.w + u.y * v.z - u.z * v.y,
             u.w * v.y - u.x * v.z + u.y * v.w + u.z * v.x, u.w * v.z + u.x * v.y - u.y * v.x + u.z * v.w};
 }
+// geom.py:224-232 qd_quat_mul_axis: q times the pure quaternion (0, a)
+__device__ __forceinline__ f4q qmul_axis(f4q q, f3 a) {
+    return {-q.x * a.x - q.y * a.y - q.z * a.z, q.w * a.x + q.y * a.z - q.z * a.y, q.w * a.y + q.z * a.x - q.x * a.z,
+            q.w * a.z + q.x * a.y - q.y * a.x};
+}
 // transform quat v by quat u: normalize(u * v)
 __device__ __forceinline__ f4q qtrans(f4q v, f4q u) { return normalize4(qmul(u, v)); }
 
@@ -394,6 +399,6 @@ constexpr int CTRL_POSITION = 0, CTRL_VELOCITY = 1, CTRL_FORCE = 2;
 constexpr int JT_FIXED = 0, JT_REVOLUTE = 1, JT_PRISMATIC = 2, JT_SPHERICAL = 3, JT_FREE = 4;
 constexpr int GT_PLANE = 0, GT_SPHERE = 1, GT_ELLIPSOID = 2, GT_CYLINDER = 3, GT_CAPSULE = 4, GT_BOX = 5,
               GT_MESH = 6, GT_TERRAIN = 7;
-constexpr int EQ_JOINT = 2;
+constexpr int EQ_CONNECT = 0, EQ_WELD = 1, EQ_JOINT = 2;
 
 }  // namespace rs

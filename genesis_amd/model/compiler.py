@@ -34,6 +34,7 @@ from .specs import (
     JOINT_PRISMATIC,
     JOINT_REVOLUTE,
     JOINT_SPHERICAL,
+    EQ_CONNECT,
     EQ_JOINT,
     EQ_WELD,
     EntitySpec,
@@ -143,8 +144,9 @@ def compile_model(entities: list[EntitySpec], options: SolverOptions | None = No
         d1 = sum(j.n_dofs for (_, j) in joints)
         ent_rows.append((l0, len(links), d0, d1, g0, len(geoms), ent.gravity_compensation, ent.is_local_collision_mask))
         name_to_joint = {joint_names[k]: k for k in range(j0, len(joints))}
+        name_to_link = {link_names[k]: k for k in range(l0, len(links))}
         for eq in ent.equalities:
-            eq_rows.append((eq, name_to_joint))
+            eq_rows.append((eq, name_to_joint, name_to_link))
 
     L, J, G, E = len(links), len(joints), len(geoms), len(entities)
     D = sum(j.n_dofs for (_, j) in joints)
@@ -292,11 +294,17 @@ def compile_model(entities: list[EntitySpec], options: SolverOptions | None = No
     eq2 = np.zeros(NE, np.int64)
     eqdata = np.zeros((NE, 11))
     eqsp = np.zeros((NE, 7))
-    for k, (eq, name_to_joint) in enumerate(eq_rows):
+    for k, (eq, name_to_joint, name_to_link) in enumerate(eq_rows):
         eqtype[k] = eq.type
-        if eq.type != EQ_JOINT:
-            raise NotImplementedError("only joint equalities are compiled")
-        eq1[k], eq2[k] = name_to_joint[eq.obj1], name_to_joint[eq.obj2]
+        if eq.type == EQ_JOINT:
+            eq1[k], eq2[k] = name_to_joint[eq.obj1], name_to_joint[eq.obj2]
+        elif eq.type in (EQ_CONNECT, EQ_WELD):
+            for nm in (eq.obj1, eq.obj2):
+                if nm not in name_to_link:
+                    raise ValueError(f"equality: no link named {nm!r} in its entity (merged into its parent?)")
+            eq1[k], eq2[k] = name_to_link[eq.obj1], name_to_link[eq.obj2]
+        else:
+            raise NotImplementedError(f"equality type {eq.type}")
         eqdata[k] = eq.data
         eqsp[k] = eq.sol_params
     if NE:
@@ -348,6 +356,9 @@ def compile_model(entities: list[EntitySpec], options: SolverOptions | None = No
         n_links=L, n_joints=J, n_dofs=D, n_qs=Q, n_geoms=G, n_entities=E, n_equalities=NE, arrays=A, options=opt,
         link_names=link_names, joint_names=joint_names, entity_names=[e.name for e in entities],
     )
+
+    # ------------------------------------------------------------------ equalities defined at qpos0
+    _resolve_equalities_at_qpos0(model, [eq for (eq, _, _) in eq_rows])
 
     # ------------------------------------------------------------------ invweight / meaninertia
     _init_invweight_and_meaninertia(model, [ls.invweight for (_, _, ls) in links])
@@ -490,6 +501,29 @@ def mass_matrix(model: CompiledModel, qpos: np.ndarray):
     M = np.tril(M) + np.tril(M, -1).T
     M[np.diag_indices(D)] += A["dof_armature"]
     return M, i_pos, cdof_ang, cdof_vel
+
+
+def _resolve_equalities_at_qpos0(model: CompiledModel, specs):
+    """Connect: the anchor's coordinates in the second link's frame; weld: the anchor in the first link's frame and,
+    when not given, the second link's orientation relative to the first - all at qpos0, so that the constraint holds
+    there."""
+    if not any(eq.at_qpos0 for eq in specs):
+        return
+    A = model.arrays
+    lpos, lquat = fk_com_cdof(model, A["qpos0"])[:2]
+    for k, eq in enumerate(specs):
+        if not eq.at_qpos0:
+            continue
+        l1, l2 = int(A["eq_obj1"][k]), int(A["eq_obj2"][k])
+        d = A["eq_data"][k]
+        if eq.type == EQ_CONNECT:
+            world = tf.transform_point(d[0:3], lpos[l1], lquat[l1])
+            d[3:6] = tf.quat_to_R(lquat[l2]).T @ (world - lpos[l2])
+        elif eq.type == EQ_WELD:
+            world = tf.transform_point(d[0:3], lpos[l2], lquat[l2])
+            d[3:6] = tf.quat_to_R(lquat[l1]).T @ (world - lpos[l1])
+            if not np.any(d[6:10]):
+                d[6:10] = tf.quat_normalize(tf.quat_mul(tf.quat_inv(lquat[l1]), lquat[l2]))
 
 
 def _init_invweight_and_meaninertia(model: CompiledModel, given_link_invweight):

@@ -23,7 +23,9 @@ from . import meshes as mu
 from . import transforms as tf
 from .specs import (
     DEFAULT_SOLIMP,
+    EQ_CONNECT,
     EQ_JOINT,
+    EQ_WELD,
     GEOM_BOX,
     GEOM_CAPSULE,
     GEOM_CYLINDER,
@@ -384,23 +386,34 @@ class MJCFLoader:
         if eq is None:
             return out
         for e in eq:
-            if e.tag != "joint":
-                raise NotImplementedError(f"MJCF equality '{e.tag}' is not supported")
-            poly = _floats(e.attrib.get("polycoef", "0 1 0 0 0"), 5)
             solref = _floats(e.attrib.get("solref", f"{MIN_TIMECONST} 1"))
             solimp = _floats(e.attrib.get("solimp", " ".join(map(str, DEFAULT_SOLIMP))))
             solimp = np.concatenate([solimp, np.array(DEFAULT_SOLIMP)[len(solimp) :]])
+            sol_params = np.concatenate([solref[:2], solimp[:5]])
             data = np.zeros(11)
-            data[:5] = poly[:5]
-            out.append(
-                EqualitySpec(
-                    type=EQ_JOINT,
-                    obj1=e.attrib["joint1"],
-                    obj2=e.attrib["joint2"],
-                    data=data,
-                    sol_params=np.concatenate([solref[:2], solimp[:5]]),
-                )
-            )
+            if e.tag == "joint":
+                poly = _floats(e.attrib.get("polycoef", "0 1 0 0 0"), 5)
+                data[:5] = poly[:5]
+                out.append(EqualitySpec(type=EQ_JOINT, obj1=e.attrib["joint1"], obj2=e.attrib["joint2"], data=data,
+                                        sol_params=sol_params))
+            elif e.tag in ("connect", "weld"):
+                # body-to-body only (mjcf.py parse_equalities takes the compiled MuJoCo eq_data: the second anchor,
+                # and a weld's unset relative pose, follow from qpos0)
+                if "body1" not in e.attrib or "body2" not in e.attrib:
+                    raise NotImplementedError(f"MJCF {e.tag}: only body1/body2 equalities are supported")
+                data[0:3] = _floats(e.attrib.get("anchor", "0 0 0"), 3)
+                if e.tag == "connect":
+                    etype = EQ_CONNECT
+                else:
+                    etype = EQ_WELD
+                    relpose = _floats(e.attrib.get("relpose", "0 1 0 0 0 0 0"), 7)
+                    if np.any(relpose[3:7]) and not np.allclose(relpose, [0, 1, 0, 0, 0, 0, 0]):
+                        raise NotImplementedError("MJCF weld: an explicit relpose is not supported")
+                    data[10] = float(_floats(e.attrib.get("torquescale", "1"), 1)[0])
+                out.append(EqualitySpec(type=etype, obj1=e.attrib["body1"], obj2=e.attrib["body2"], data=data,
+                                        sol_params=sol_params, at_qpos0=True))
+            else:
+                raise NotImplementedError(f"MJCF equality '{e.tag}' is not supported")
         return out
 
     # ------------------------------------------------------------- actuators

@@ -87,11 +87,19 @@ class LinkSpec:
 
 @dataclass
 class EqualitySpec:
+    """An equality constraint (solver.py:689-938). JOINT: obj1/obj2 are joint names and data[0:5] the polynomial
+    coefficients. CONNECT: obj1/obj2 are link names, data[0:3] the anchor in obj1's frame and data[3:6] the same point
+    in obj2's frame. WELD: data[0:3] the anchor in obj2's frame, data[3:6] in obj1's frame, data[6:10] the pose of
+    obj2 relative to obj1 (quaternion), data[10] the torque scale. at_qpos0: fill data[3:6] (and, for a weld whose
+    data[6:10] is zero, the relative pose) from the configuration at qpos0, so that the constraint holds there (what
+    MuJoCo's model compiler does for MJCF equalities)."""
+
     type: int
     obj1: str
     obj2: str
     data: np.ndarray  # 11 floats
     sol_params: np.ndarray
+    at_qpos0: bool = False
 
 
 @dataclass

@@ -66,6 +66,14 @@ QUADRUPED_STAND = np.array([0.0, 0.0, 0.0, 0.0, 0.7, 0.7, 0.7, 0.7, -1.4, -1.4, 
 QUADRUPED_BASE_Z = 0.36
 
 
+CLOSED_LOOPS_XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets", "xml", "closed_loops.xml")
+
+
+def closed_loops_model(**opts):
+    """A four-bar linkage closed by a connect equality and a free box welded to a pendulum (no collisions), dt 0.005."""
+    return compile_model([load_mjcf(CLOSED_LOOPS_XML, "loops")], SolverOptions(dt=0.005, **opts))
+
+
 def quadruped_model(**opts):
     return compile_model([plane_entity(), load_mjcf(QUADRUPED_XML, "quadruped")], SolverOptions(dt=0.005, **opts))
 

@@ -5,39 +5,179 @@
 
 namespace orc {
 
-// solver.py:801-878 func_equality_joint; :881-938 add_equality_constraints (joint equalities only)
-template <class T> void add_equality_constraints(const Model<T>& m, Env<T>& s) {
+// jacobian row of a point p attached to `link`, component i of sign * (cdof_vel - (p - root_COM) x cdof_ang), added
+// into row c along the link's chain (dofs of each link last to first); returns the row's contribution to J v.
+// with_rot: the velocity goes through qd_transform_motion_by_trans_quat with the identity rotation (connect rows,
+// solver.py:757-760), else it is taken as is (weld rows, :1086-1089).
+template <class T>
+T point_jac_row(const Model<T>& m, Env<T>& s, int c, int link, V3<T> p, T sign, int i, bool with_rot) {
+    using E = Env<T>;
     const int D = m.D;
-    s.n_con = 0;
-    s.n_con_eq = 0;
-    for (int i_e = 0; i_e < m.NE; i_e++) {
-        if (m.eq_type[i_e] != EQ_JOINT) continue;
-        const T* sp = &m.eq_sol_params[7 * i_e];
-        const T* data = &m.eq_data[11 * i_e];
-        int j1 = m.eq_obj1[i_e], j2 = m.eq_obj2[i_e];
-        int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
-        int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
+    T jq = 0;
+    while (link > -1) {
+        for (int dd = 0; dd < m.link_dof_end[link] - m.link_dof_start[link]; dd++) {
+            int k = m.link_dof_end[link] - 1 - dd;
+            V3<T> ca = E::g3(s.cdof_ang, k), cv = E::g3(s.cdof_vel, k);
+            V3<T> tpos = p - E::g3(s.root_COM, link);
+            V3<T> vel = cv - cross(tpos, ca);
+            if (with_rot) vel = transform_by_quat(vel, identity_quat<T>());
+            V3<T> diff = sign * vel;
+            T j = diff[i];
+            jq = jq + j * s.vel[k];
+            s.jac[(size_t)c * D + k] = s.jac[(size_t)c * D + k] + j;
+        }
+        link = m.link_parent[link];
+    }
+    return jq;
+}
+
+// solver.py:689-797 func_equality_connect: three rows holding the two anchors together
+template <class T> void equality_connect(const Model<T>& m, Env<T>& s, int i_e) {
+    using E = Env<T>;
+    const int D = m.D;
+    const T* sp = &m.eq_sol_params[7 * i_e];
+    const T* data = &m.eq_data[11 * i_e];
+    const int l1 = m.eq_obj1[i_e], l2 = m.eq_obj2[i_e];
+    V3<T> a1{data[0], data[1], data[2]}, a2{data[3], data[4], data[5]};
+    V3<T> g1 = transform_by_trans_quat(a1, E::g3(s.links_pos, l1), E::g4(s.links_quat, l1));
+    V3<T> g2 = transform_by_trans_quat(a2, E::g3(s.links_pos, l2), E::g4(s.links_quat, l2));
+    T invw = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+    for (int i3 = 0; i3 < 3; i3++) {
         int c = s.n_con++;
         s.n_con_eq++;
         for (int d = 0; d < D; d++) s.jac[(size_t)c * D + d] = T(0);
-        T diff = s.qpos[q2] - m.qpos0[q2];
-        T pos = s.qpos[q1] - m.qpos0[q1];
-        T deriv = 0;
-        for (int i5 = 0; i5 < 5; i5++) {
-            T dp = std::pow(diff, T(i5));
-            pos = pos - dp * data[i5];
-            if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * T(i5 + 1);
+        T jq = 0;
+        for (int i_ab = 0; i_ab < 2; i_ab++) {
+            T sign = i_ab == 0 ? T(1) : T(-1);
+            jq = jq + point_jac_row(m, s, c, i_ab == 0 ? l1 : l2, i_ab == 0 ? g1 : g2, sign, i3, true);
         }
-        s.jac[(size_t)c * D + d1] = T(1);
-        s.jac[(size_t)c * D + d2] = -deriv;
-        T jac_qvel = s.jac[(size_t)c * D + d1] * s.vel[d1] + s.jac[(size_t)c * D + d2] * s.vel[d2];
-        T invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+        V3<T> pd = g1 - g2;
+        T pen = norm(pd);
         T imp, aref;
-        imp_aref(sp, -std::fabs(pos), jac_qvel, pos, imp, aref);
+        imp_aref(sp, -pen, jq, pd[i3], imp, aref);
         T dg = std::fmax(invw * (T(1) - imp) / imp, m.EPS);
         s.diag[c] = dg;
         s.aref[c] = aref;
         s.efc_D[c] = T(1) / dg;
+    }
+}
+
+// solver.py:984-1173 func_equality_weld: three position rows, then three orientation rows
+template <class T> void equality_weld(const Model<T>& m, Env<T>& s, int i_e) {
+    using E = Env<T>;
+    const int D = m.D;
+    const T* sp = &m.eq_sol_params[7 * i_e];
+    const T* data = &m.eq_data[11 * i_e];
+    const int l1 = m.eq_obj1[i_e], l2 = m.eq_obj2[i_e];
+    V3<T> a1{data[3], data[4], data[5]}, a2{data[0], data[1], data[2]};
+    Q4<T> relpose{data[6], data[7], data[8], data[9]};
+    const T ts = data[10];
+    Q4<T> q1 = E::g4(s.links_quat, l1), q2 = E::g4(s.links_quat, l2);
+    V3<T> g1 = transform_by_trans_quat(a1, E::g3(s.links_pos, l1), q1);
+    V3<T> g2 = transform_by_trans_quat(a2, E::g3(s.links_pos, l2), q2);
+    V3<T> pe = g1 - g2;
+    Q4<T> q = quat_mul(q1, relpose);
+    Q4<T> iq2 = inv_quat(q2);
+    Q4<T> eq = quat_mul(iq2, q);
+    V3<T> re = V3<T>{eq.x, eq.y, eq.z} * ts;
+    T all[6] = {pe.x, pe.y, pe.z, re.x, re.y, re.z};
+    T sq = 0;
+    for (int k = 0; k < 6; k++) sq = sq + all[k] * all[k];
+    const T pos_imp = std::sqrt(sq);
+    const T invw0 = m.link_invweight[2 * l1] + m.link_invweight[2 * l2];
+    const T invw1 = m.link_invweight[2 * l1 + 1] + m.link_invweight[2 * l2 + 1];
+    for (int i = 0; i < 3; i++) {
+        int c = s.n_con++;
+        s.n_con_eq++;
+        for (int d = 0; d < D; d++) s.jac[(size_t)c * D + d] = T(0);
+        T jq = 0;
+        for (int i_ab = 0; i_ab < 2; i_ab++) {
+            T sign = i_ab == 0 ? T(1) : T(-1);
+            jq = jq + point_jac_row(m, s, c, i_ab == 0 ? l1 : l2, i_ab == 0 ? g1 : g2, sign, i, false);
+        }
+        T imp, aref;
+        imp_aref(sp, -pos_imp, jq, pe[i], imp, aref);
+        T dg = std::fmax(invw0 * (T(1) - imp) / imp, m.EPS);
+        s.diag[c] = dg;
+        s.aref[c] = aref;
+        s.efc_D[c] = T(1) / dg;
+    }
+    const int c0 = s.n_con;
+    s.n_con += 3;
+    s.n_con_eq += 3;
+    for (int k = 0; k < 3; k++)
+        for (int d = 0; d < D; d++) s.jac[(size_t)(c0 + k) * D + d] = T(0);
+    for (int i_ab = 0; i_ab < 2; i_ab++) {
+        T sign = i_ab == 0 ? T(1) : T(-1);
+        int link = i_ab == 0 ? l1 : l2;
+        while (link > -1) {
+            for (int dd = 0; dd < m.link_dof_end[link] - m.link_dof_start[link]; dd++) {
+                int d = m.link_dof_end[link] - 1 - dd;
+                V3<T> jv = sign * E::g3(s.cdof_ang, d);
+                for (int k = 0; k < 3; k++) s.jac[(size_t)(c0 + k) * D + d] = s.jac[(size_t)(c0 + k) * D + d] + jv[k];
+            }
+            link = m.link_parent[link];
+        }
+    }
+    T jq3[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+        V3<T> jd{s.jac[(size_t)c0 * D + d], s.jac[(size_t)(c0 + 1) * D + d], s.jac[(size_t)(c0 + 2) * D + d]};
+        Q4<T> quat2 = quat_mul_axis(iq2, jd);
+        Q4<T> quat3 = quat_mul(quat2, q);
+        for (int k = 0; k < 3; k++) {
+            s.jac[(size_t)(c0 + k) * D + d] = T(0.5) * quat3[k + 1] * ts;
+            jq3[k] = jq3[k] + s.jac[(size_t)(c0 + k) * D + d] * s.vel[d];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        T imp, aref;
+        imp_aref(sp, -pos_imp, jq3[k], re[k], imp, aref);
+        T dg = std::fmax(invw1 * (T(1) - imp) / imp, m.EPS);
+        s.diag[c0 + k] = dg;
+        s.aref[c0 + k] = aref;
+        s.efc_D[c0 + k] = T(1) / dg;
+    }
+}
+
+// solver.py:801-878 func_equality_joint
+template <class T> void equality_joint(const Model<T>& m, Env<T>& s, int i_e) {
+    const int D = m.D;
+    const T* sp = &m.eq_sol_params[7 * i_e];
+    const T* data = &m.eq_data[11 * i_e];
+    int j1 = m.eq_obj1[i_e], j2 = m.eq_obj2[i_e];
+    int q1 = m.joint_q_start[j1], q2 = m.joint_q_start[j2];
+    int d1 = m.joint_dof_start[j1], d2 = m.joint_dof_start[j2];
+    int c = s.n_con++;
+    s.n_con_eq++;
+    for (int d = 0; d < D; d++) s.jac[(size_t)c * D + d] = T(0);
+    T diff = s.qpos[q2] - m.qpos0[q2];
+    T pos = s.qpos[q1] - m.qpos0[q1];
+    T deriv = 0;
+    for (int i5 = 0; i5 < 5; i5++) {
+        T dp = std::pow(diff, T(i5));
+        pos = pos - dp * data[i5];
+        if (i5 < 4) deriv = deriv + data[i5 + 1] * dp * T(i5 + 1);
+    }
+    s.jac[(size_t)c * D + d1] = T(1);
+    s.jac[(size_t)c * D + d2] = -deriv;
+    T jac_qvel = s.jac[(size_t)c * D + d1] * s.vel[d1] + s.jac[(size_t)c * D + d2] * s.vel[d2];
+    T invw = m.dof_invweight[d1] + m.dof_invweight[d2];
+    T imp, aref;
+    imp_aref(sp, -std::fabs(pos), jac_qvel, pos, imp, aref);
+    T dg = std::fmax(invw * (T(1) - imp) / imp, m.EPS);
+    s.diag[c] = dg;
+    s.aref[c] = aref;
+    s.efc_D[c] = T(1) / dg;
+}
+
+// solver.py:881-938 add_equality_constraints: the rows of every equality, in equality order
+template <class T> void add_equality_constraints(const Model<T>& m, Env<T>& s) {
+    s.n_con = 0;
+    s.n_con_eq = 0;
+    for (int i_e = 0; i_e < m.NE; i_e++) {
+        if (m.eq_type[i_e] == EQ_CONNECT) equality_connect(m, s, i_e);
+        else if (m.eq_type[i_e] == EQ_WELD) equality_weld(m, s, i_e);
+        else if (m.eq_type[i_e] == EQ_JOINT) equality_joint(m, s, i_e);
     }
 }
 

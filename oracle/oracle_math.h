@@ -57,6 +57,11 @@ template <class T> inline Q4<T> quat_mul(Q4<T> u, Q4<T> v) {
 }
 // geom.py:246-252: quat_mul(u, v) normalised ("transform v by u")
 template <class T> inline Q4<T> transform_quat_by_quat(Q4<T> v, Q4<T> u) { return normalized(quat_mul(u, v)); }
+// geom.py:224-232 qd_quat_mul_axis: q times the pure quaternion (0, a)
+template <class T> inline Q4<T> quat_mul_axis(Q4<T> q, V3<T> a) {
+    return {-q.x * a.x - q.y * a.y - q.z * a.z, q.w * a.x + q.y * a.z - q.z * a.y, q.w * a.y + q.z * a.x - q.x * a.z,
+            q.w * a.z + q.x * a.y - q.y * a.x};
+}
 
 // geom.py:256-270 (divides by |q|^2)
 template <class T> inline V3<T> transform_by_quat(V3<T> v, Q4<T> q) {
