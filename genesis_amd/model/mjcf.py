@@ -222,6 +222,25 @@ class MJCFLoader:
                 link.geoms.append(spec)
         return link
 
+    def _fromto(self, g, cls, size, pos, quat):
+        """A geom given by `fromto`: centred on the segment, its z axis along it, half-length size[1] (set in
+        place); else (pos, quat) unchanged."""
+        fromto = self._attr(g, "geom", cls, "fromto")
+        if fromto is None:
+            return pos, quat
+        ft = _floats(fromto)
+        a, b = ft[:3], ft[3:]
+        d = b - a
+        size[1] = 0.5 * np.linalg.norm(d)
+        z = np.array([0.0, 0.0, 1.0])
+        dn = d / np.linalg.norm(d)
+        c = np.cross(z, dn)
+        if np.linalg.norm(c) < 1e-12:
+            q = np.array([1.0, 0.0, 0.0, 0.0]) if dn[2] > 0 else np.array([0.0, 1.0, 0.0, 0.0])
+        else:
+            q = tf.axis_angle_quat(c, np.arctan2(np.linalg.norm(c), np.dot(z, dn)))
+        return 0.5 * (a + b), q
+
     def _inertia_from_geoms(self, b, cls):
         """MuJoCo's inertiafromgeom: every geom of the body contributes density x volume (or its mass attribute)
         and its inertia about the body frame; the sum is diagonalised into a principal inertial frame."""
@@ -232,6 +251,8 @@ class MJCFLoader:
             size = _floats(self._attr(g, "geom", cls, "size", "0 0 0"), 3)
             pos = _floats(self._attr(g, "geom", cls, "pos", "0 0 0"))
             quat = self._orientation(g, "geom", cls)
+            if tname in ("capsule", "cylinder", "box", "ellipsoid"):
+                pos, quat = self._fromto(g, cls, size, pos, quat)
             if tname == "box":
                 a, bb, c = size
                 vol = 8.0 * a * bb * c
@@ -309,20 +330,8 @@ class MJCFLoader:
         size = _floats(self._attr(g, "geom", cls, "size", "0 0 0"), 3)
         pos = _floats(self._attr(g, "geom", cls, "pos", "0 0 0"))
         quat = self._orientation(g, "geom", cls)
-        fromto = self._attr(g, "geom", cls, "fromto")
-        if fromto is not None and gtype in (GEOM_CAPSULE, GEOM_CYLINDER, GEOM_BOX, GEOM_ELLIPSOID):
-            ft = _floats(fromto)
-            a, b = ft[:3], ft[3:]
-            pos = 0.5 * (a + b)
-            d = b - a
-            size[1] = 0.5 * np.linalg.norm(d)
-            z = np.array([0.0, 0.0, 1.0])
-            dn = d / np.linalg.norm(d)
-            c = np.cross(z, dn)
-            if np.linalg.norm(c) < 1e-12:
-                quat = np.array([1.0, 0.0, 0.0, 0.0]) if dn[2] > 0 else np.array([0.0, 1.0, 0.0, 0.0])
-            else:
-                quat = tf.axis_angle_quat(c, np.arctan2(np.linalg.norm(c), np.dot(z, dn)))
+        if gtype in (GEOM_CAPSULE, GEOM_CYLINDER, GEOM_BOX, GEOM_ELLIPSOID):
+            pos, quat = self._fromto(g, cls, size, pos, quat)
         friction = float(_floats(self._attr(g, "geom", cls, "friction", "1 0.005 0.0001"))[0])
         solref = _floats(self._attr(g, "geom", cls, "solref", f"{MIN_TIMECONST} 1"))
         solimp = _floats(self._attr(g, "geom", cls, "solimp", " ".join(map(str, DEFAULT_SOLIMP))))
