@@ -3,6 +3,7 @@
 // solver.py:3164-3190) with the exact piecewise-quadratic line search (solver.py:2427-2597).
 #pragma once
 #include "rs_device.h"
+#include "rs_dyn.h"
 
 namespace rs {
 
@@ -455,11 +456,16 @@ __device__ inline bool dev_cholesky_incremental(const DModel& m, const S& st, in
     return false;
 }
 
-// solver.py:2722-2751 + :1893-1910
+// solver.py:2722-2751 + :1893-1910; the conjugate-gradient solver preconditions with the mass matrix instead
+// (solver.py:2737-2747, func_solve_mass_batch)
 template <class S>
 __device__ inline void dev_gradient(const DModel& m, const S& st, int b) {
     const int D = m.D;
     for (int d = 0; d < D; d++) SF(grad, d) = SF(Ma, d) - SF(force, d) - SF(qfrc_constraint, d);
+    if (m.solver_type == RS_SOLVER_CG) {
+        dev_solve_mass(m, st, b, FV(st, grad, b), FV(st, Mgrad, b));
+        return;
+    }
     for (int i = 0; i < D; i++) {
         float cur = SF(grad, i);
         for (int j = 0; j < i; j++) cur = cur - SF(nt_H, i * D + j) * SF(Mgrad, j);
@@ -716,7 +722,8 @@ __device__ inline void dev_solve(const DModel& m, const S& st, int b) {
     float cost = dev_update_constraint(m, st, b, SF(cost, 0));
     SF(cost, 0) = cost;
     int improved = nc > 0;
-    dev_hessian_cholesky(m, st, b);
+    const bool cg = m.solver_type == RS_SOLVER_CG;
+    if (!cg) dev_hessian_cholesky(m, st, b);
     dev_gradient(m, st, b);
     for (int d = 0; d < D; d++) SF(search, d) = -SF(Mgrad, d);
     int n_iter = 0;
@@ -733,17 +740,33 @@ __device__ inline void dev_solve(const DModel& m, const S& st, int b) {
                     SF(Ma, d) = SF(Ma, d) + SF(mv, d) * alpha;
                 }
                 for (int c = 0; c < nc; c++) SF(Jaref, c) = SF(Jaref, c) + SF(jv, c) * alpha;
+                if (cg)
+                    for (int d = 0; d < D; d++) {
+                        SF(cg_prev_grad, d) = SF(grad, d);
+                        SF(cg_prev_Mgrad, d) = SF(Mgrad, d);
+                    }
                 float prev = SF(cost, 0);
                 cost = dev_update_constraint(m, st, b, prev);
                 SF(cost, 0) = cost;
-                if (dev_cholesky_incremental(m, st, b)) dev_hessian_cholesky(m, st, b);
+                if (!cg && dev_cholesky_incremental(m, st, b)) dev_hessian_cholesky(m, st, b);
                 dev_gradient(m, st, b);
                 float gn = 0.f;
                 for (int d = 0; d < D; d++) gn = gn + SF(grad, d) * SF(grad, d);
                 gn = sqrtf(gn);
                 improved = gn > tol && (prev - cost) > tol;
-                if (improved)
+                if (improved && !cg) {
                     for (int d = 0; d < D; d++) SF(search, d) = -SF(Mgrad, d);
+                } else if (improved) {  // solver.py:2865-2884 Polak-Ribiere direction
+                    float beta = 0.f, pg_dot_pMg = 0.f;
+                    for (int d = 0; d < D; d++) {
+                        beta = beta + SF(grad, d) * (SF(Mgrad, d) - SF(cg_prev_Mgrad, d));
+                        pg_dot_pMg = pg_dot_pMg + SF(cg_prev_Mgrad, d) * SF(cg_prev_grad, d);
+                    }
+                    beta = fmaxf(beta / fmaxf(m.eps, pg_dot_pMg), 0.f);
+                    SF(cg_pg_dot_pMg, 0) = pg_dot_pMg;
+                    SF(cg_beta, 0) = beta;
+                    for (int d = 0; d < D; d++) SF(search, d) = -SF(Mgrad, d) + beta * SF(search, d);
+                }
             }
             if (!improved) break;
         }

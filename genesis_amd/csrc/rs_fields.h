@@ -7,8 +7,9 @@
 // SAP endpoints) come first and their planes are packed back to back, so that one environment's
 // local fields form a single strided sequence: local row k of environment b is element k * n_envs + b of the slab.
 // The fused kernel stages exactly that sequence into its private region. Fields sized by broadphase pairs, pairs,
-// contacts and constraint rows follow (RS_GLOBAL_FIELDS), then wk, the env-major working set of the fused kernel, and sched,
-// its cost-ordered environment queues.
+// contacts and constraint rows follow (RS_GLOBAL_FIELDS), with the conjugate-gradient solver's state (outside the
+// private region, which then stays the same size under the Newton solver), then wk, the env-major working set of
+// the fused kernel, and sched, its cost-ordered environment queues.
 #pragma once
 
 #define RS_LOCAL_FIELDS(RS_SF)               \
@@ -121,6 +122,10 @@
     RS_SF(efc_force, 0, RS_N_CONSTRAINTS, 1)         \
     RS_SF(active, 1, RS_N_CONSTRAINTS, 1)            \
     RS_SF(prev_active, 1, RS_N_CONSTRAINTS, 1)       \
+    RS_SF(cg_prev_grad, 0, RS_N_DOFS_PLANE, 1)       \
+    RS_SF(cg_prev_Mgrad, 0, RS_N_DOFS_PLANE, 1)      \
+    RS_SF(cg_beta, 0, RS_N_ONE_PLANE, 1)             \
+    RS_SF(cg_pg_dot_pMg, 0, RS_N_ONE_PLANE, 1)       \
     RS_SF(wk, 0, RS_N_WORK, 1)                       \
     RS_SF(sched, 1, RS_N_SCHED, 1)
 

@@ -106,6 +106,8 @@ static int64_t count_of(const rs_model_header& h, int kind) {
         case RS_N_SORT: return 2 * (int64_t)h.n_geoms;
         case RS_N_WORK: return work_floats(h);
         case RS_N_SCHED: return sched_rows();
+        case RS_N_DOFS_PLANE: return h.n_dofs;
+        case RS_N_ONE_PLANE: return 1;
         default: return 0;
     }
 }
@@ -450,9 +452,9 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
 #define RS_X(name, is_int, kind, width) d.name = (decltype(d.name))(base + h->array_offset[i++]);
     RS_MODEL_ARRAYS(RS_X)
 #undef RS_X
-    if (h->solver_type != 1) {
+    if (h->solver_type != RS_SOLVER_CG && h->solver_type != RS_SOLVER_NEWTON) {
         rs_model_destroy(m);
-        return fail(RS_E_UNSUPPORTED, "rs_model_create: only the Newton constraint solver is built");
+        return fail(RS_E_INVALID, "rs_model_create: solver_type must be RS_SOLVER_CG or RS_SOLVER_NEWTON");
     }
     if (h->integrator != 2) {
         rs_model_destroy(m);

@@ -260,6 +260,15 @@ __device__ inline void wave_gradient(const DModel& m, const LState& loc) {
     }
 }
 
+// solver.py:2722-2751 with the conjugate-gradient solver's preconditioner, Mgrad = M^-1 grad (solver.py:2737-2747,
+// func_solve_mass_batch; forward_dynamics.py:813-894), uniform across the wave in the reference's order
+__device__ inline void wave_gradient_cg(const DModel& m, const LState& loc) {
+    const int D = m.D;
+    for (int d = 0; d < D; d++) LF(grad, d) = LF(Ma, d) - LF(force, d) - LF(qfrc_constraint, d);
+    dev_solve_mass(m, loc, 0, FV(loc, grad, 0), FV(loc, Mgrad, 0));
+    wave_sync();
+}
+
 // solver.py:2722-2751 + :1893-1910 for models with up to 128 dofs, lane-parallel: lane l holds rows l and l + 64.
 // Forward substitution column by column (row i subtracts L[i][j] x_j for j = 0, 1, ... in turn, the reference's
 // order for row i); backward substitution row by row, row i subtracting L[j][i] x_j for j = i + 1, i + 2, ... in
@@ -975,7 +984,8 @@ __device__ inline float wave_linesearch(const DModel& m, const LState& loc, cons
     return res;
 }
 
-// solver.py:2936-3059 (init), :3063-3139 (Newton iteration), :3164-3190 (body), :3241-3261, :3199-3237
+// solver.py:2936-3059 (init), :3063-3139 (Newton or conjugate-gradient iteration), :3164-3190 (body), :3241-3261,
+// :3199-3237
 template <int KD>
 __device__ inline void wave_solve(const DModel& m, const LState& loc, const DState& st, int b, const WorkRows& R,
                                   int lane, float* tile) {
@@ -999,11 +1009,16 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     else cost = wave_update_constraint(m, loc, R, lane, LF(cost, 0));
     LF(cost, 0) = cost;
     int improved = nc > 0;
-    if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
-    else wave_hessian_cholesky(m, loc, R, lane);
-    if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
-    else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
-    else wave_gradient(m, loc);
+    const bool cg = m.solver_type == RS_SOLVER_CG;
+    if (cg) {
+        wave_gradient_cg(m, loc);
+    } else {
+        if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
+        else wave_hessian_cholesky(m, loc, R, lane);
+        if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+        else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
+        else wave_gradient(m, loc);
+    }
     for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
     int n_iter = 0;
     if (nc > 0) {
@@ -1019,31 +1034,47 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
                     LF(Ma, d) = LF(Ma, d) + LF(mv, d) * alpha;
                 }
                 for (int c = lane; c < nc; c += kWave) R.Jaref(c) = R.Jaref(c) + R.jv(c) * alpha;
+                if (cg)
+                    for (int d = 0; d < D; d++) {
+                        FV(st, cg_prev_grad, b)[d] = LF(grad, d);
+                        FV(st, cg_prev_Mgrad, b)[d] = LF(Mgrad, d);
+                    }
                 wave_sync();
                 float prev = LF(cost, 0);
-                bool degenerate;
-                if constexpr (KD > 0) {
-                    cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev, tile);
-                    LF(cost, 0) = cost;
-                    degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane, tile);
+                if constexpr (KD > 0) cost = wave_update_constraint_r<KD>(m, loc, R, lane, prev, tile);
+                else cost = wave_update_constraint(m, loc, R, lane, prev);
+                LF(cost, 0) = cost;
+                if (cg) {
+                    wave_gradient_cg(m, loc);
                 } else {
-                    cost = wave_update_constraint(m, loc, R, lane, prev);
-                    LF(cost, 0) = cost;
-                    degenerate = wave_cholesky_incremental(m, loc, R, lane);
+                    bool degenerate;
+                    if constexpr (KD > 0) degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane, tile);
+                    else degenerate = wave_cholesky_incremental(m, loc, R, lane);
+                    if (degenerate) {
+                        if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
+                        else wave_hessian_cholesky(m, loc, R, lane);
+                    }
+                    if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
+                    else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
+                    else wave_gradient(m, loc);
                 }
-                if (degenerate) {
-                    if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
-                    else wave_hessian_cholesky(m, loc, R, lane);
-                }
-                if constexpr (KD > 0) wave_gradient_r<KD>(m, loc, lane);
-                else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
-                else wave_gradient(m, loc);
                 float gn = 0.f;
                 for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
                 gn = sqrtf(gn);
                 improved = gn > tol && (prev - cost) > tol;
-                if (improved)
+                if (improved && !cg) {
                     for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+                } else if (improved) {  // solver.py:2865-2884 Polak-Ribiere direction
+                    float beta = 0.f, pg_dot_pMg = 0.f;
+                    for (int d = 0; d < D; d++) {
+                        beta = beta + LF(grad, d) * (LF(Mgrad, d) - FV(st, cg_prev_Mgrad, b)[d]);
+                        pg_dot_pMg = pg_dot_pMg + FV(st, cg_prev_Mgrad, b)[d] * FV(st, cg_prev_grad, b)[d];
+                    }
+                    beta = fmaxf(beta / fmaxf(m.eps, pg_dot_pMg), 0.f);
+                    FV(st, cg_pg_dot_pMg, b)[0] = pg_dot_pMg;
+                    FV(st, cg_beta, b)[0] = beta;
+                    for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d) + beta * LF(search, d);
+                }
             }
             if (!improved) break;
         }

@@ -31,7 +31,8 @@ ERR_INVALID_ACC_NAN = 1 << 3
 
 COUNT_KINDS = [
     "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
-    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK", "SCHED",
+    "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK", "SCHED", "DOFS_PLANE",
+    "ONE_PLANE",
 ]
 ENV_MAJOR_KINDS = {"WORK"}
 LOCAL_KINDS = {"ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "DOFS2", "SORT"}

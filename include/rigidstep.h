@@ -61,7 +61,7 @@ enum {
     RS_STAGE_BROADPHASE = 5,  /* sweep-and-prune (collider/broadphase.py:141-396) */
     RS_STAGE_NARROWPHASE = 6, /* contacts (collider/narrowphase.py:1988-2189) */
     RS_STAGE_CONSTRAINTS = 7, /* equality, friction-loss, contact and limit rows (constraint/solver.py:580-1286) */
-    RS_STAGE_SOLVE = 8,       /* Newton solve + qacc (constraint/solver.py:2936-3261) */
+    RS_STAGE_SOLVE = 8,       /* Newton or CG solve + qacc (constraint/solver.py:2936-3261) */
     RS_STAGE_INTEGRATE = 9,   /* cacc, semi-implicit Euler, NaN guard (rigid_solver.py:2783-2891) */
     RS_STAGE_COUNT = 10,
 };
@@ -75,10 +75,16 @@ enum {
     RS_N_BROAD, RS_N_SORT,
     RS_N_WORK,  /* per-environment working set of the fused substep (size from rs_state_field_rows) */
     RS_N_SCHED, /* fused-substep schedule: 2 x 4 cost-class queues of environment ids and one control row */
+    RS_N_DOFS_PLANE, /* n_dofs, always a [row][env] plane (state the fused substep keeps outside its private region) */
+    RS_N_ONE_PLANE,  /* 1, likewise */
     RS_N_KINDS
 };
 
 #define RS_MAX_MODEL_ARRAYS 96
+
+/* constraint solvers (genesis/constants.py:59-61 constraint_solver) */
+#define RS_SOLVER_CG 0
+#define RS_SOLVER_NEWTON 1
 
 typedef struct rs_model_header {
     int32_t abi_version;     /* RS_ABI_VERSION */
@@ -95,7 +101,7 @@ typedef struct rs_model_header {
     int32_t ls_iterations;
     float ls_tolerance;
     int32_t enable_collision, enable_joint_limit, disable_constraint, enable_multi_contact, box_box_detection;
-    int32_t integrator, solver_type;
+    int32_t integrator, solver_type;                /* solver_type: RS_SOLVER_CG or RS_SOLVER_NEWTON */
     float meaninertia;
     float mc_perturbation, mc_tolerance, mpr_to_gjk_overlap_ratio;
     int32_t n_arrays;                                /* number of directory entries used */

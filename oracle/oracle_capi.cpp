@@ -63,6 +63,7 @@ template <class T> struct Context {
         R(contact_force);
         R(jac); R(diag); R(aref); R(efc_D); R(efc_frictionloss); R(Jaref); R(jv); R(efc_force); R(active);
         R(prev_active); R(qacc); R(Ma); R(grad); R(Mgrad); R(search); R(mv); R(qfrc_constraint); R(nt_H);
+        R(cg_prev_grad); R(cg_prev_Mgrad);
 #undef R
         reg_scalar("is_warmstart", &Env<T>::is_warmstart);
         reg_scalar("errno", &Env<T>::errno_);
@@ -75,6 +76,8 @@ template <class T> struct Context {
         reg_scalar("improved", &Env<T>::improved);
         reg_scalar("n_iter", &Env<T>::n_iter);
         reg_scalar("cost", &Env<T>::cost);
+        reg_scalar("cg_beta", &Env<T>::cg_beta);
+        reg_scalar("cg_pg_dot_pMg", &Env<T>::cg_pg_dot_pMg);
     }
 
     void init_env(Env<T>& s) {

@@ -1,6 +1,7 @@
 // ORACLE (test infrastructure only): constraint rows and the Newton solver of one environment.
 // Restates genesis/engine/solvers/rigid/constraint/solver.py (the batch/CPU code path, "monolith" solve body).
 #pragma once
+#include "oracle_dynamics.h"
 #include "oracle_model.h"
 
 namespace orc {
@@ -388,6 +389,10 @@ template <class T> bool hessian_cholesky_incremental(const Model<T>& m, Env<T>& 
 template <class T> void update_gradient(const Model<T>& m, Env<T>& s) {
     const int D = m.D;
     for (int d = 0; d < D; d++) s.grad[d] = s.Ma[d] - s.force[d] - s.qfrc_constraint[d];
+    if (m.h.solver_type == SOLVER_CG) {  // :2737-2747 Mgrad = M^-1 grad (func_solve_mass_batch)
+        solve_mass(m, s, s.grad, s.Mgrad);
+        return;
+    }
     for (int i = 0; i < D; i++) {
         T cur = s.grad[i];
         for (int j = 0; j < i; j++) cur = cur - s.nt_H[i * D + j] * s.Mgrad[j];
@@ -598,7 +603,7 @@ template <class T> T linesearch(const Model<T>& m, Env<T>& s) {
     return res;
 }
 
-// solver.py:2936-3059 func_solve_init, :3063-3139 func_solve_iter (Newton), :3164-3190 monolith body,
+// solver.py:2936-3059 func_solve_init, :3063-3139 func_solve_iter (Newton or CG), :3164-3190 monolith body,
 // :3241-3261 func_update_qacc, :3199-3237 func_update_contact_force
 template <class T> void constraint_solve(const Model<T>& m, Env<T>& s) {
     const int D = m.D;
@@ -616,7 +621,8 @@ template <class T> void constraint_solve(const Model<T>& m, Env<T>& s) {
     }
     update_constraint(m, s, s.qacc, s.Ma, s.cost);
     s.improved = s.n_con > 0;
-    hessian_cholesky_direct(m, s);
+    const bool cg = m.h.solver_type == SOLVER_CG;
+    if (!cg) hessian_cholesky_direct(m, s);
     update_gradient(m, s);
     for (int d = 0; d < D; d++) s.search[d] = -s.Mgrad[d];
     s.n_iter = 0;
@@ -632,8 +638,13 @@ template <class T> void constraint_solve(const Model<T>& m, Env<T>& s) {
                     s.Ma[d] = s.Ma[d] + s.mv[d] * alpha;
                 }
                 for (int c = 0; c < s.n_con; c++) s.Jaref[c] = s.Jaref[c] + s.jv[c] * alpha;
+                if (cg)
+                    for (int d = 0; d < D; d++) {
+                        s.cg_prev_grad[d] = s.grad[d];
+                        s.cg_prev_Mgrad[d] = s.Mgrad[d];
+                    }
                 update_constraint(m, s, s.qacc, s.Ma, s.cost);
-                if (hessian_cholesky_incremental(m, s)) hessian_cholesky_direct(m, s);
+                if (!cg && hessian_cholesky_incremental(m, s)) hessian_cholesky_direct(m, s);
                 update_gradient(m, s);
                 T tol = (T(m.meaninertia) * T(std::max(1, D))) * T(m.tolerance);
                 T improvement = s.prev_cost - s.cost;
@@ -641,8 +652,19 @@ template <class T> void constraint_solve(const Model<T>& m, Env<T>& s) {
                 for (int d = 0; d < D; d++) gn = gn + s.grad[d] * s.grad[d];
                 gn = std::sqrt(gn);
                 s.improved = gn > tol && improvement > tol;
-                if (s.improved)
+                if (s.improved && !cg) {
                     for (int d = 0; d < D; d++) s.search[d] = -s.Mgrad[d];
+                } else if (s.improved) {  // :2865-2884 Polak-Ribiere direction
+                    T beta = 0, pg_dot_pMg = 0;
+                    for (int d = 0; d < D; d++) {
+                        beta = beta + s.grad[d] * (s.Mgrad[d] - s.cg_prev_Mgrad[d]);
+                        pg_dot_pMg = pg_dot_pMg + s.cg_prev_Mgrad[d] * s.cg_prev_grad[d];
+                    }
+                    beta = std::fmax(beta / std::fmax(m.EPS, pg_dot_pMg), T(0));
+                    s.cg_pg_dot_pMg = pg_dot_pMg;
+                    s.cg_beta = beta;
+                    for (int d = 0; d < D; d++) s.search[d] = -s.Mgrad[d] + beta * s.search[d];
+                }
             }
             if (!s.improved) break;
         }

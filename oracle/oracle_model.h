@@ -16,6 +16,7 @@ constexpr int JT_FIXED = 0, JT_REVOLUTE = 1, JT_PRISMATIC = 2, JT_SPHERICAL = 3,
 constexpr int GT_PLANE = 0, GT_SPHERE = 1, GT_ELLIPSOID = 2, GT_CYLINDER = 3, GT_CAPSULE = 4, GT_BOX = 5, GT_MESH = 6,
               GT_TERRAIN = 7;
 constexpr int EQ_CONNECT = 0, EQ_WELD = 1, EQ_JOINT = 2;
+constexpr int SOLVER_CG = 0, SOLVER_NEWTON = 1;
 
 
 template <class T> struct Model {
@@ -119,8 +120,8 @@ template <class T> struct Env {
     int32_t n_con = 0, n_con_eq = 0, n_con_fl = 0, improved = 0, ls_it = 0, ls_result = 0, n_iter = 0;
     std::vector<T> jac, diag, aref, efc_D, efc_frictionloss, Jaref, jv, efc_force;
     std::vector<int32_t> active, prev_active;
-    std::vector<T> qacc, Ma, grad, Mgrad, search, mv, qfrc_constraint, nt_H, nt_vec;
-    T cost = 0, prev_cost = 0, gauss = 0, gtol = 0;
+    std::vector<T> qacc, Ma, grad, Mgrad, search, mv, qfrc_constraint, nt_H, nt_vec, cg_prev_grad, cg_prev_Mgrad;
+    T cost = 0, prev_cost = 0, gauss = 0, gtol = 0, cg_beta = 0, cg_pg_dot_pMg = 0;
     T quad_gauss[3] = {0, 0, 0}, eq_sum[3] = {0, 0, 0};
 
     void allocate(const Model<T>& m) {
@@ -150,6 +151,7 @@ template <class T> struct Env {
         efc_frictionloss.assign(C, 0); Jaref.assign(C, 0); jv.assign(C, 0); efc_force.assign(C, 0);
         active.assign(C, 0); prev_active.assign(C, 0);
         qacc.assign(D, 0); Ma.assign(D, 0); grad.assign(D, 0); Mgrad.assign(D, 0); search.assign(D, 0);
+        cg_prev_grad.assign(D, 0); cg_prev_Mgrad.assign(D, 0);
         mv.assign(D, 0); qfrc_constraint.assign(D, 0); nt_H.assign(D * D, 0); nt_vec.assign(D, 0);
     }
 
