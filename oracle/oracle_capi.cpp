@@ -70,6 +70,8 @@ template <class T> struct Context {
         reg_scalar("first_time", &Env<T>::first_time);
         reg_scalar("n_broad", &Env<T>::n_broad);
         reg_scalar("n_contacts", &Env<T>::n_contacts);
+        reg_scalar("n_gjk_regime", &Env<T>::n_gjk_regime);
+        reg_scalar("n_gjk_regime_perturbed", &Env<T>::n_gjk_regime_perturbed);
         reg_scalar("n_con", &Env<T>::n_con);
         reg_scalar("n_con_eq", &Env<T>::n_con_eq);
         reg_scalar("n_con_fl", &Env<T>::n_con_fl);

@@ -31,6 +31,8 @@ template <class T> void broadphase_sap(const Model<T>& m, Env<T>& s) {
         for (int k = 0; k < 3; k++) s.contact_pos[3 * i + k] = s.contact_normal[3 * i + k] = s.contact_force[3 * i + k] = 0;
     }
     s.n_contacts = 0;
+    s.n_gjk_regime = 0;
+    s.n_gjk_regime_perturbed = 0;
     const int G = m.G;
     if (s.first_time) {
         int ib = 0;
@@ -601,6 +603,13 @@ template <class T> void convex_convex_contact(const Model<T>& m, Env<T>& s, int 
                         mpr_contact(m, ga, gb, nws, pa, qa, pb, qb, is_col, n, pen, cp);
                         updated = true;
                     }
+                }
+                // narrowphase.py:729-734: a deep first penetration (no cached direction) or a large overlap would go
+                // to GJK/EPA in the reference; only counted here
+                if (pen > tolerance &&
+                    (!avail || T(m.h.mc_tolerance) * pen >= T(m.h.mpr_to_gjk_overlap_ratio) * tolerance)) {
+                    if (det == 0) s.n_gjk_regime++;
+                    else s.n_gjk_regime_perturbed++;
                 }
             }
         }

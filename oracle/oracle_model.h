@@ -112,6 +112,9 @@ template <class T> struct Env {
     std::vector<T> cdd_ang, cdd_vel, cfrc_ang, cfrc_vel, cacc_ang, cacc_lin;
     // collision
     int32_t n_broad = 0, n_contacts = 0;
+    // MPR detections this step that the reference would hand to GJK/EPA (narrowphase.py:729-734; not built: the MPR
+    // result is kept), first detections and perturbed ones
+    int32_t n_gjk_regime = 0, n_gjk_regime_perturbed = 0;
     std::vector<int32_t> broad_pairs, active_buffer;
     std::vector<int32_t> contact_geom_a, contact_geom_b, contact_link_a, contact_link_b, contact_pair;
     std::vector<T> contact_pos, contact_normal, contact_penetration, contact_friction, contact_sol_params,
