@@ -262,7 +262,7 @@ __device__ inline void dev_mass_matrix(const DModel& m, const S& st, int b) {
     }
     for (int d = 0; d < D; d++) {
         float v = SF(mass_mat, d * D + d) + m.dof_armature[d];
-        if (m.integrator == 2) {
+        if (m.integrator == RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) {
             v = v + m.dof_damping[d] * m.dt;
             if (SF(ctrl_mode, d) <= CTRL_VELOCITY) v = v - m.dof_act_bias[3 * d + 2] * m.dt;
         }
@@ -270,14 +270,22 @@ __device__ inline void dev_mass_matrix(const DModel& m, const S& st, int b) {
     }
 }
 
-// forward_dynamics.py:560-602 func_factor_mass (reverse-order L^T D L)
+// forward_dynamics.py:560-602 func_factor_mass (reverse-order L^T D L) of entity e; with implicit_damping (:575-588,
+// called from func_implicit_damping) the diagonal gains damping*dt, and under implicitfast -kv*dt for dofs in position
+// or velocity control
 template <class S>
-__device__ inline void dev_factor_mass(const DModel& m, const S& st, int b) {
+__device__ inline void dev_factor_mass_entity(const DModel& m, const S& st, int b, int e, bool implicit_damping) {
     const int D = m.D;
-    for (int e = 0; e < m.E; e++) {
+    {
         int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
-        for (int i = d0; i < d1; i++)
+        for (int i = d0; i < d1; i++) {
             for (int j = d0; j <= i; j++) SF(mass_L, i * D + j) = SF(mass_mat, i * D + j);
+            if (implicit_damping) {
+                SF(mass_L, i * D + i) = SF(mass_L, i * D + i) + m.dof_damping[i] * m.dt;
+                if (m.integrator == RS_INTEGRATOR_IMPLICITFAST && SF(ctrl_mode, i) <= CTRL_VELOCITY)
+                    SF(mass_L, i * D + i) = SF(mass_L, i * D + i) - m.dof_act_bias[3 * i + 2] * m.dt;
+            }
+        }
         for (int ii = 0; ii < d1 - d0; ii++) {
             int i = d1 - ii - 1;
             float dinv = 1.f / SF(mass_L, i * D + i);
@@ -289,6 +297,43 @@ __device__ inline void dev_factor_mass(const DModel& m, const S& st, int b) {
                 SF(mass_L, i * D + j) = a;
             }
             SF(mass_L, i * D + i) = 1.f;
+        }
+    }
+}
+
+template <class S>
+__device__ inline void dev_factor_mass(const DModel& m, const S& st, int b) {
+    for (int e = 0; e < m.E; e++) dev_factor_mass_entity(m, st, b, e, false);
+}
+
+// forward_dynamics.py:1788-1859 func_implicit_damping (integrators Euler and implicitfast, without MuJoCo
+// compatibility): every entity with a damped dof, or under implicitfast a position- or velocity-controlled dof with a
+// velocity gain, re-factors M + diag(damping*dt [- kv*dt]) and takes acc = that matrix^-1 force; the others keep acc
+template <class S>
+__device__ inline void dev_implicit_damping(const DModel& m, const S& st, int b) {
+    const int D = m.D;
+    for (int e = 0; e < m.E; e++) {
+        const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
+        bool mask = false;
+        for (int d = d0; d < d1; d++) {
+            if (m.dof_damping[d] > m.eps) mask = true;
+            if (m.integrator != RS_INTEGRATOR_EULER && SF(ctrl_mode, d) <= CTRL_VELOCITY &&
+                fabsf(m.dof_act_bias[3 * d + 2]) > m.eps)
+                mask = true;
+        }
+        if (!mask) continue;
+        dev_factor_mass_entity(m, st, b, e, true);
+        for (int ii = 0; ii < d1 - d0; ii++) {
+            int i = d1 - ii - 1;
+            float cur = SF(force, i);
+            for (int j = i + 1; j < d1; j++) cur = cur - SF(mass_L, j * D + i) * SF(acc, j);
+            SF(acc, i) = cur;
+        }
+        for (int i = d0; i < d1; i++) SF(acc, i) = SF(acc, i) * SF(mass_D_inv, i);
+        for (int i = d0; i < d1; i++) {
+            float cur = SF(acc, i);
+            for (int j = d0; j < i; j++) cur = cur - SF(mass_L, i * D + j) * SF(acc, j);
+            SF(acc, i) = cur;
         }
     }
 }

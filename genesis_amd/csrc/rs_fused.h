@@ -337,6 +337,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     }
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
     wave_update_acc(m, loc, lane, true);
+    if (m.integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) dev_implicit_damping(m, loc, 0);
     dev_integrate(m, loc, 0);
     wave_sync();
     wave_kinematics(m, loc, lane);

@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(kBlock) k_stage(DModel m, DState st) {
         case RS_STAGE_SOLVE: dev_solve(m, st, b); break;
         case RS_STAGE_INTEGRATE:
             dev_update_acc(m, st, b, true);
+            if (m.integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) dev_implicit_damping(m, st, b);
             dev_integrate(m, st, b);
             dev_kinematics(m, st, b, false);
             break;
@@ -456,9 +457,10 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
         rs_model_destroy(m);
         return fail(RS_E_INVALID, "rs_model_create: solver_type must be RS_SOLVER_CG or RS_SOLVER_NEWTON");
     }
-    if (h->integrator != 2) {
+    if (h->integrator != RS_INTEGRATOR_EULER && h->integrator != RS_INTEGRATOR_IMPLICITFAST &&
+        h->integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) {
         rs_model_destroy(m);
-        return fail(RS_E_UNSUPPORTED, "rs_model_create: only the approximate_implicitfast integrator is built");
+        return fail(RS_E_INVALID, "rs_model_create: integrator must be one of RS_INTEGRATOR_*");
     }
     static_assert(k_n_state_fields <= kMaxStateFields, "FusedArgs::loc_off too small");
     fused_layout(m->h, &m->fa);

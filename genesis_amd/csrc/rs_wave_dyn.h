@@ -53,7 +53,7 @@ __device__ inline void wave_mass_matrix(const DModel& m, const LState& loc, int 
                   m.mass_parent_mask[a * D + c];
         if (i == j) {
             v = v + m.dof_armature[i];
-            if (m.integrator == 2) {
+            if (m.integrator == RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) {
                 v = v + m.dof_damping[i] * m.dt;
                 if (LF(ctrl_mode, i) <= CTRL_VELOCITY) v = v - m.dof_act_bias[3 * i + 2] * m.dt;
             }

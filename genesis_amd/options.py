@@ -72,7 +72,6 @@ class RigidOptions(_Options):
 
     def validate(self):
         unsupported = {
-            "integrator": (self.integrator, integrator.approximate_implicitfast),
             "noslip_iterations": (self.noslip_iterations, 0),
             "use_contact_island": (self.use_contact_island, False),
             "use_hibernation": (self.use_hibernation, False),
@@ -82,6 +81,8 @@ class RigidOptions(_Options):
             "batch_dofs_info": (self.batch_dofs_info, False),
             "use_gjk_collision": (bool(self.use_gjk_collision), False),
         }
+        if self.integrator not in tuple(integrator):
+            raise ValueError(f"RigidOptions.integrator={self.integrator!r} is not an integrator")
         if self.constraint_solver not in (constraint_solver.CG, constraint_solver.Newton):
             raise ValueError(f"RigidOptions.constraint_solver={self.constraint_solver!r} is not a constraint solver")
         for name, (value, supported) in unsupported.items():

@@ -82,6 +82,11 @@ enum {
 
 #define RS_MAX_MODEL_ARRAYS 96
 
+/* integrators (genesis/constants.py:52-55 integrator) */
+#define RS_INTEGRATOR_EULER 0
+#define RS_INTEGRATOR_IMPLICITFAST 1
+#define RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST 2
+
 /* constraint solvers (genesis/constants.py:59-61 constraint_solver) */
 #define RS_SOLVER_CG 0
 #define RS_SOLVER_NEWTON 1
@@ -101,7 +106,7 @@ typedef struct rs_model_header {
     int32_t ls_iterations;
     float ls_tolerance;
     int32_t enable_collision, enable_joint_limit, disable_constraint, enable_multi_contact, box_box_detection;
-    int32_t integrator, solver_type;                /* solver_type: RS_SOLVER_CG or RS_SOLVER_NEWTON */
+    int32_t integrator, solver_type;                /* RS_INTEGRATOR_*, RS_SOLVER_CG or RS_SOLVER_NEWTON */
     float meaninertia;
     float mc_perturbation, mc_tolerance, mpr_to_gjk_overlap_ratio;
     int32_t n_arrays;                                /* number of directory entries used */

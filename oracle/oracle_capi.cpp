@@ -99,7 +99,7 @@ template <class T> struct Context {
 
     // One substep (rigid_solver.py:912-980), with kinematics refreshed at the end like kernel_step_2.
     void step_env(Env<T>& s) {
-        const bool implicit = m.h.integrator == 2;
+        const bool implicit = m.h.integrator == RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST;
         compute_mass_matrix(m, s, implicit);
         factor_mass(m, s);
         torque_and_passive(m, s);
@@ -119,6 +119,7 @@ template <class T> struct Context {
             constraint_solve(m, s);
         }
         update_acc(m, s, true);
+        if (m.h.integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) implicit_damping(m, s);
         integrate(m, s);
         forward_kinematics(m, s);
         com_links(m, s);
@@ -127,7 +128,7 @@ template <class T> struct Context {
     }
 
     void stage(Env<T>& s, int st) {
-        const bool implicit = m.h.integrator == 2;
+        const bool implicit = m.h.integrator == RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST;
         switch (st) {
             case RS_STAGE_FK:
                 forward_kinematics(m, s); com_links(m, s); update_geoms(m, s, false); forward_velocity(m, s);
@@ -145,7 +146,9 @@ template <class T> struct Context {
                 break;
             case RS_STAGE_SOLVE: if (!m.h.disable_constraint) constraint_solve(m, s); break;
             case RS_STAGE_INTEGRATE:
-                update_acc(m, s, true); integrate(m, s);
+                update_acc(m, s, true);
+                if (m.h.integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) implicit_damping(m, s);
+                integrate(m, s);
                 forward_kinematics(m, s); com_links(m, s); update_geoms(m, s, false); forward_velocity(m, s);
                 break;
         }

@@ -259,14 +259,23 @@ template <class T> void compute_mass_matrix(const Model<T>& m, Env<T>& s, bool i
     }
 }
 
-// forward_dynamics.py:560-602 func_factor_mass (serial reverse-order L^T D L)
-template <class T> void factor_mass(const Model<T>& m, Env<T>& s) {
+// forward_dynamics.py:560-602 func_factor_mass (serial reverse-order L^T D L) of entity i_e; with implicit_damping
+// (:575-588, called from func_implicit_damping) the diagonal gains damping*dt, and under implicitfast -kv*dt for
+// dofs in position or velocity control
+template <class T> void factor_mass_entity(const Model<T>& m, Env<T>& s, int i_e, bool implicit_damping) {
     const int D = m.D;
-    for (int i_e = 0; i_e < m.E; i_e++) {
+    {
         int d0 = m.entity_dof_start[i_e], d1 = m.entity_dof_end[i_e];
         int nd = d1 - d0;
-        for (int i = d0; i < d1; i++)
+        for (int i = d0; i < d1; i++) {
             for (int j = d0; j <= i; j++) s.mass_L[i * D + j] = s.mass_mat[i * D + j];
+            if (implicit_damping) {
+                const T dt = T(m.dt);
+                s.mass_L[i * D + i] = s.mass_L[i * D + i] + m.dof_damping[i] * dt;
+                if (m.h.integrator == RS_INTEGRATOR_IMPLICITFAST && s.ctrl_mode[i] <= CTRL_VELOCITY)
+                    s.mass_L[i * D + i] = s.mass_L[i * D + i] - m.dof_act_bias[3 * i + 2] * dt;
+            }
+        }
         for (int ii = 0; ii < nd; ii++) {
             int i = d1 - ii - 1;
             T D_inv = T(1) / s.mass_L[i * D + i];
@@ -280,6 +289,10 @@ template <class T> void factor_mass(const Model<T>& m, Env<T>& s) {
             s.mass_L[i * D + i] = T(1);
         }
     }
+}
+
+template <class T> void factor_mass(const Model<T>& m, Env<T>& s) {
+    for (int i_e = 0; i_e < m.E; i_e++) factor_mass_entity(m, s, i_e, false);
 }
 
 // forward_dynamics.py:813-894 func_solve_mass_entity
@@ -298,6 +311,37 @@ template <class T> void solve_mass(const Model<T>& m, const Env<T>& s, const std
             T cur = out[i];
             for (int j = d0; j < i; j++) cur = cur - s.mass_L[i * D + j] * out[j];
             out[i] = cur;
+        }
+    }
+}
+
+// forward_dynamics.py:1788-1859 func_implicit_damping (integrators Euler and implicitfast, without MuJoCo
+// compatibility): every entity with a damped dof, or under implicitfast a position- or velocity-controlled dof with a
+// velocity gain, re-factors M + diag(damping*dt [- kv*dt]) and takes acc = that matrix^-1 force; the others keep acc
+template <class T> void implicit_damping(const Model<T>& m, Env<T>& s) {
+    const int D = m.D;
+    for (int i_e = 0; i_e < m.E; i_e++) {
+        const int d0 = m.entity_dof_start[i_e], d1 = m.entity_dof_end[i_e];
+        bool mask = false;
+        for (int d = d0; d < d1; d++) {
+            if (m.dof_damping[d] > m.EPS) mask = true;
+            if (m.h.integrator != RS_INTEGRATOR_EULER && s.ctrl_mode[d] <= CTRL_VELOCITY &&
+                std::fabs(m.dof_act_bias[3 * d + 2]) > m.EPS)
+                mask = true;
+        }
+        if (!mask) continue;
+        factor_mass_entity(m, s, i_e, true);
+        for (int ii = 0; ii < d1 - d0; ii++) {
+            int i = d1 - ii - 1;
+            T cur = s.force[i];
+            for (int j = i + 1; j < d1; j++) cur = cur - s.mass_L[j * D + i] * s.acc[j];
+            s.acc[i] = cur;
+        }
+        for (int i = d0; i < d1; i++) s.acc[i] = s.acc[i] * s.mass_D_inv[i];
+        for (int i = d0; i < d1; i++) {
+            T cur = s.acc[i];
+            for (int j = d0; j < i; j++) cur = cur - s.mass_L[i * D + j] * s.acc[j];
+            s.acc[i] = cur;
         }
     }
 }
