@@ -7,6 +7,8 @@ per-environment error bits polled every ``RATE_CHECK_ERRNO`` substeps (``simulat
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 import torch
 
@@ -21,6 +23,24 @@ RATE_CHECK_ERRNO = 10
 
 class GenesisException(Exception):
     pass
+
+
+# state fields that are not per-environment physics state: the fused kernel's scratch and its batch schedule
+_NOT_STATE_KINDS = ("WORK", "SCHED")
+CTRL_FORCE = 2
+
+
+@dataclass
+class SimState:
+    """Snapshot of every environment's rigid state (``RigidSolverState``, ``rigid_solver.py:1430-1456``): generalized
+    positions, velocities and accelerations and link poses, batch first."""
+
+    t: int
+    qpos: torch.Tensor  # [B, n_qs]
+    dofs_vel: torch.Tensor  # [B, n_dofs]
+    dofs_acc: torch.Tensor  # [B, n_dofs]
+    links_pos: torch.Tensor  # [B, n_links, 3]
+    links_quat: torch.Tensor  # [B, n_links, 4]
 
 
 class Scene:
@@ -135,6 +155,13 @@ class Scene:
         return self.slab["errbits"][0] > 0
 
     def reset(self, state=None, envs_idx=None):
+        """Back to the state at build (``scene.py:958``), or to ``state`` from :meth:`get_state`, for every
+        environment or those in ``envs_idx``."""
+        if state is not None:
+            self._set_state(state, envs_idx)
+            if envs_idx is None:
+                self.t = state.t
+            return
         mask = self._env_mask(envs_idx)
         slab = self.slab
         if mask is None:
@@ -143,11 +170,77 @@ class Scene:
             fresh = runtime.StateSlab(self.dmodel, self._B, self.device)
             m = mask.bool()
             for (name, _, kind, _), t in zip(slab.directory, slab.fields.values()):
-                if kind in ("WORK", "SCHED"):  # fused-kernel scratch and batch-level schedule: not per-env state
+                if kind in _NOT_STATE_KINDS:
                     continue
                 t[:, m] = fresh.fields[name][:, m]
             del fresh
         self.t = 0
+
+    def get_state(self) -> SimState:
+        """``RigidSolver.get_state`` (``rigid_solver.py:1430-1456``): a copy of the state to come back to."""
+        B, L = self._B, self.model.n_links
+        S = self.slab
+        return SimState(
+            t=self.t,
+            qpos=S["qpos"].t().clone(),
+            dofs_vel=S["vel"].t().clone(),
+            dofs_acc=S["acc"].t().clone(),
+            links_pos=S["links_pos"].t().reshape(B, L, 3).clone(),
+            links_quat=S["links_quat"].t().reshape(B, L, 4).clone(),
+        )
+
+    def _set_state(self, state: SimState, envs_idx=None):
+        """``RigidSolver.set_state`` (``rigid_solver.py:1458-1580``, not partial): the environments take the
+        snapshot's positions, velocities, accelerations and link poses, their controls are cleared to zero force,
+        their contact caches and warm starts and error bits are cleared, and their kinematics are recomputed."""
+        B = self._B
+        S = self.slab
+        idx = self._env_index(envs_idx)
+        cols = slice(None) if idx is None else idx
+
+        def put(name, src):
+            f = S[name]
+            src = torch.as_tensor(src, device=self.device).reshape(B, -1).to(f.dtype)
+            if src.shape[1] != f.shape[0]:
+                raise GenesisException(f"state field {name!r}: {src.shape[1]} values per environment, "
+                                       f"expected {f.shape[0]}")
+            if idx is None:
+                f.copy_(src.t())
+            else:
+                f[:, idx] = src[idx].t()
+
+        put("qpos", state.qpos)
+        put("vel", state.dofs_vel)
+        put("acc", state.dofs_acc)
+        put("links_pos", state.links_pos)
+        put("links_quat", state.links_quat)
+        S["ctrl_force"][:, cols] = 0.0
+        S["ctrl_mode"][:, cols] = CTRL_FORCE
+        for name in ("n_contacts", "n_con", "n_con_eq", "n_con_fl"):  # collider and constraint-solver clear
+            S[name][:, cols] = 0
+        self._after_state_write(envs_idx, skip_forward=False)
+
+    def save_checkpoint(self, path) -> None:
+        """``scene.py:1402-1417``: every environment's full state to one file (numpy ``.npz``: field name ->
+        [rows, n_envs] array, plus the step index), from which :meth:`load_checkpoint` resumes bit for bit."""
+        arrays = {"step_index": np.array(self.t)}
+        for (name, _, kind, _), t in zip(self.slab.directory, self.slab.fields.values()):
+            if kind not in _NOT_STATE_KINDS:
+                arrays[name] = t.cpu().numpy()
+        with open(path, "wb") as f:
+            np.savez(f, **arrays)
+
+    def load_checkpoint(self, path) -> None:
+        """``scene.py:1419-1446``: restore a file written by :meth:`save_checkpoint` for the same scene."""
+        with np.load(path, allow_pickle=False) as z:
+            for (name, _, kind, _), t in zip(self.slab.directory, self.slab.fields.values()):
+                if kind in _NOT_STATE_KINDS:
+                    continue
+                if name not in z.files or z[name].shape != tuple(t.shape):
+                    raise GenesisException(f"checkpoint {path!s} does not hold field {name!r} of shape "
+                                           f"{tuple(t.shape)}: it was saved from another scene")
+                t.copy_(torch.as_tensor(z[name], device=self.device))
+            self.t = int(z["step_index"])
 
     # ------------------------------------------------------------------ field helpers
     def _env_index(self, envs_idx):

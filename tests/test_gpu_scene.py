@@ -92,3 +92,64 @@ def test_contacts_and_errno(gs):
     with pytest.raises(gs.GenesisException):
         for _ in range(60):
             scene2.step()
+
+
+def _stepped_franka(gs, B, n):
+    from genesis_amd.scenes import franka_initial_qpos, franka_model, franka_targets
+
+    scene, franka = _franka_scene(gs, B)
+    m = franka_model()
+    franka.set_dofs_position(torch.as_tensor(franka_initial_qpos(m, B), dtype=torch.float32, device="cuda"))
+    tgt = torch.as_tensor(franka_targets(m, B, 0), dtype=torch.float32, device="cuda")
+    franka.control_dofs_position(tgt)
+    for _ in range(n):
+        scene.step()
+    return scene, franka, tgt
+
+
+def test_get_state_and_reset_to_it(gs):
+    """RigidSolver.get_state / set_state (rigid_solver.py:1430-1580): the snapshot's positions, velocities and link
+    poses come back exactly, controls are cleared to zero force, and a partial reset leaves other envs untouched."""
+    B = 16
+    print("[scene] get_state / reset(state)", flush=True)
+    scene, franka, tgt = _stepped_franka(gs, B, 20)
+    state = scene.get_state()
+    for _ in range(10):
+        scene.step()
+    moved_q = franka.get_dofs_position().clone()
+    scene.reset(state, envs_idx=[1, 3])
+    q = franka.get_qpos()
+    keep = torch.ones(B, dtype=torch.bool, device="cuda")
+    keep[[1, 3]] = False
+    assert torch.equal(q[~keep], state.qpos[~keep])
+    assert torch.equal(franka.get_dofs_position()[keep], moved_q[keep])
+    scene.reset(state)
+    assert torch.equal(franka.get_qpos(), state.qpos)
+    assert torch.equal(franka.get_dofs_velocity(), state.dofs_vel)
+    # link poses recomputed from the restored configuration equal the snapshot's
+    assert torch.equal(scene.slab["links_pos"].t().reshape(B, -1, 3), state.links_pos)
+    assert torch.equal(scene.slab["links_quat"].t().reshape(B, -1, 4), state.links_quat)
+    assert int(scene.slab["ctrl_mode"].min()) == 2 and float(scene.slab["ctrl_force"].abs().max()) == 0.0
+    franka.control_dofs_position(tgt)
+    for _ in range(5):
+        scene.step()
+    assert torch.isfinite(franka.get_dofs_position()).all()
+
+
+def test_checkpoint_resumes_bit_for_bit(gs, tmp_path):
+    """scene.py:1402-1446 save_checkpoint / load_checkpoint: resuming from the file replays the same substeps to the
+    last bit (warm starts, contact caches and sweep-and-prune order included)."""
+    B = 32
+    print("[scene] save_checkpoint / load_checkpoint", flush=True)
+    scene, franka, _ = _stepped_franka(gs, B, 20)
+    path = tmp_path / "franka.ckpt"
+    scene.save_checkpoint(path)
+    for _ in range(10):
+        scene.step()
+    want_q, want_v = franka.get_qpos().clone(), franka.get_dofs_velocity().clone()
+    scene.load_checkpoint(path)
+    assert scene.t == 20
+    for _ in range(10):
+        scene.step()
+    assert torch.equal(franka.get_qpos(), want_q)
+    assert torch.equal(franka.get_dofs_velocity(), want_v)
