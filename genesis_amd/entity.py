@@ -96,6 +96,49 @@ class RigidEntity:
     def get_links_quat(self, links_idx_local=None, envs_idx=None):
         return self._scene._get_vec("links_quat", self._links(links_idx_local), 4, envs_idx)
 
+    def get_links_vel(self, links_idx_local=None, envs_idx=None, *, ref="link_origin"):
+        """Linear velocity of each link at its origin, its centre of mass or the root COM
+        (``rigid_solver.py:2345-2372``): ``cd_vel + cd_ang x delta`` from the spatial velocity about the root COM."""
+        links = self._links(links_idx_local)
+        sc = self._scene
+        cd_vel = sc._get_vec("cd_vel", links, 3, envs_idx)
+        if ref == "root_com":
+            return cd_vel
+        if ref == "link_com":
+            delta = sc._get_vec("i_pos", links, 3, envs_idx)
+        elif ref == "link_origin":
+            delta = sc._get_vec("links_pos", links, 3, envs_idx) - sc._get_vec("root_COM", links, 3, envs_idx)
+        else:
+            raise ValueError(f"ref must be 'link_origin', 'link_com' or 'root_com', not {ref!r}")
+        return cd_vel + torch.cross(sc._get_vec("cd_ang", links, 3, envs_idx), delta, dim=-1)
+
+    def get_links_ang(self, links_idx_local=None, envs_idx=None):
+        """Angular velocity of each link (``cd_ang``, ``rigid_solver.py:2374-2382``)."""
+        return self._scene._get_vec("cd_ang", self._links(links_idx_local), 3, envs_idx)
+
+    def _dof_array(self, name, dofs_idx_local, column=None):
+        a = self._scene.model.arrays[name][self._dofs(dofs_idx_local).cpu().numpy()]
+        if column is not None:
+            a = a[:, column]
+        return torch.as_tensor(a, dtype=torch.float32, device=self._scene.device)
+
+    def get_dofs_kp(self, dofs_idx_local=None, envs_idx=None):
+        """Position gains of the PD controller (``rigid_entity.py:3678``)."""
+        return self._dof_array("dof_act_gain", dofs_idx_local)
+
+    def get_dofs_kv(self, dofs_idx_local=None, envs_idx=None):
+        """Velocity gains of the PD controller (``rigid_entity.py:3698``): ``-act_bias[2]``."""
+        return -self._dof_array("dof_act_bias", dofs_idx_local, 2)
+
+    def get_dofs_force_range(self, dofs_idx_local=None, envs_idx=None):
+        """(lower, upper) force limits (``rigid_entity.py:3742``)."""
+        lo = self._dof_array("dof_force_range", dofs_idx_local, 0)
+        return lo, self._dof_array("dof_force_range", dofs_idx_local, 1)
+
+    def get_dofs_limit(self, dofs_idx_local=None, envs_idx=None):
+        """(lower, upper) position limits (``rigid_entity.py:1701``)."""
+        return self._dof_array("dof_limit", dofs_idx_local, 0), self._dof_array("dof_limit", dofs_idx_local, 1)
+
     def get_pos(self, envs_idx=None):
         return self.get_links_pos([0], envs_idx)[..., 0, :]
 

@@ -153,3 +153,27 @@ def test_checkpoint_resumes_bit_for_bit(gs, tmp_path):
         scene.step()
     assert torch.equal(franka.get_qpos(), want_q)
     assert torch.equal(franka.get_dofs_velocity(), want_v)
+
+
+def test_link_velocities_and_dof_parameters(gs):
+    """get_links_vel (rigid_solver.py:2345-2372) against a finite difference of the link origins along the joint
+    velocities (q + h v through forward kinematics), and the PD gains / limits as set."""
+    B, h = 8, 1e-3
+    print("[scene] get_links_vel / get_dofs_kp", flush=True)
+    scene, franka, _ = _stepped_franka(gs, B, 15)
+    v_links = franka.get_links_vel()
+    assert torch.equal(franka.get_links_vel(ref="root_com"), scene._get_vec("cd_vel", franka._links(None), 3, None))
+    assert torch.equal(franka.get_links_ang(), scene._get_vec("cd_ang", franka._links(None), 3, None))
+    p0 = franka.get_links_pos().double()
+    q, v = franka.get_qpos().double(), franka.get_dofs_velocity().double()
+    franka.set_qpos((q + h * v).float(), zero_velocity=False)
+    fd = (franka.get_links_pos().double() - p0) / h
+    scale = float(v_links.abs().max())
+    assert scale > 0.1
+    assert float((fd - v_links.double()).abs().max()) < 2e-2 * scale + 5e-3
+    franka.set_dofs_kp(torch.arange(1, 10, dtype=torch.float32) * 100.0)
+    franka.set_dofs_kv(torch.full((9,), 7.0))
+    assert torch.allclose(franka.get_dofs_kp().cpu(), torch.arange(1, 10, dtype=torch.float32) * 100.0)
+    assert torch.allclose(franka.get_dofs_kv().cpu(), torch.full((9,), 7.0))
+    lo, hi = franka.get_dofs_limit()
+    assert bool((lo < hi).all())
