@@ -112,6 +112,23 @@ class RigidEntity:
             raise ValueError(f"ref must be 'link_origin', 'link_com' or 'root_com', not {ref!r}")
         return cd_vel + torch.cross(sc._get_vec("cd_ang", links, 3, envs_idx), delta, dim=-1)
 
+    def get_links_acc(self, links_idx_local=None, envs_idx=None):
+        """Classical linear acceleration of each link at its origin (``abd/accessor.py:960-983``): the spatial
+        acceleration of the last substep (``cacc``, taken before integration, as in the reference) moved to the
+        link origin, plus ``omega x v`` at the current velocities."""
+        links = self._links(links_idx_local)
+        sc = self._scene
+        cpos = sc._get_vec("links_pos", links, 3, envs_idx) - sc._get_vec("root_COM", links, 3, envs_idx)
+        acc_ang = sc._get_vec("cacc_ang", links, 3, envs_idx)
+        acc_lin = sc._get_vec("cacc_lin", links, 3, envs_idx) + torch.cross(acc_ang, cpos, dim=-1)
+        ang = sc._get_vec("cd_ang", links, 3, envs_idx)
+        vel = sc._get_vec("cd_vel", links, 3, envs_idx) + torch.cross(ang, cpos, dim=-1)
+        return acc_lin + torch.cross(ang, vel, dim=-1)
+
+    def get_links_acc_ang(self, links_idx_local=None, envs_idx=None):
+        """Angular acceleration of each link (``cacc_ang``, ``rigid_entity.py:3224``)."""
+        return self._scene._get_vec("cacc_ang", self._links(links_idx_local), 3, envs_idx)
+
     def get_links_ang(self, links_idx_local=None, envs_idx=None):
         """Angular velocity of each link (``cd_ang``, ``rigid_solver.py:2374-2382``)."""
         return self._scene._get_vec("cd_ang", self._links(links_idx_local), 3, envs_idx)
