@@ -198,6 +198,21 @@ class RigidEntity:
         """act_bias[2] = -kv (abd/accessor.py:587-601)."""
         self._scene._set_model_dofs(self._dofs(dofs_idx_local), kv, "kv")
 
+    def set_dofs_damping(self, damping, dofs_idx_local=None, envs_idx=None):
+        """Joint damping of the dofs (``rigid_entity.py:3481``): the passive force -damping*v and,
+        under approximate_implicitfast, the mass matrix's damping*dt term."""
+        self._scene._set_model_dofs(self._dofs(dofs_idx_local), damping, "damping")
+
+    def set_dofs_stiffness(self, stiffness, dofs_idx_local=None, envs_idx=None):
+        """Joint stiffness of the dofs (``rigid_entity.py:3463``): the passive force -stiffness*q."""
+        self._scene._set_model_dofs(self._dofs(dofs_idx_local), stiffness, "stiffness")
+
+    def get_dofs_damping(self, dofs_idx_local=None, envs_idx=None):
+        return self._dof_array("dof_damping", dofs_idx_local)
+
+    def get_dofs_stiffness(self, dofs_idx_local=None, envs_idx=None):
+        return self._dof_array("dof_stiffness", dofs_idx_local)
+
     def set_dofs_force_range(self, lower, upper, dofs_idx_local=None, envs_idx=None):
         d = self._dofs(dofs_idx_local)
         self._scene._set_model_dofs(d, np.stack([np.broadcast_to(np.asarray(lower, float), (len(d),)),

@@ -347,6 +347,8 @@ class Scene:
             A["dof_act_bias"][d, 2] = -np.broadcast_to(v, (len(d),))
         elif kind == "force_range":
             A["dof_force_range"][d] = v
+        elif kind in ("damping", "stiffness"):
+            A[f"dof_{kind}"][d] = np.broadcast_to(v, (len(d),))
         torch.cuda.synchronize()
         self.dmodel = runtime.DeviceModel(self.model)
         self.slab.dmodel = self.dmodel
