@@ -84,43 +84,48 @@ def make_slab(wl, dm, B):
 
 
 def run_path(wl, dm, B, steps, warmup, mode, targets, use_graph, n_lat=200):
-    """Returns (wall_s, gpu_ms, per-substep latencies ms, mean n_broad, slab)."""
+    """Returns (wall_s, gpu_ms, per-substep latencies ms, mean n_broad, slab, timed-region launch counts, graph
+    substeps).
+
+    Targets change every BLOCK substeps (counted from the first warm-up substep). The fused path replays a hipGraph
+    of min(BLOCK, steps) substeps, so the timed region runs the graph for any --steps that divides into it."""
     slab = make_slab(wl, dm, B)
     stream = torch.cuda.Stream()
+    gblock = min(BLOCK, steps)
     with torch.cuda.stream(stream):
-        graph = rt.StepGraph(slab, BLOCK, mode, stream) if use_graph else None
+        graph = rt.StepGraph(slab, gblock, mode, stream) if use_graph else None
+        state = {"t": 0, "graph_launches": 0, "eager_substeps": 0}
 
-        def do_block(k, n):
-            if targets is not None:
-                slab["ctrl_pos"].copy_(targets[k % len(targets)])
-            if graph is not None and n == BLOCK:
-                graph.launch()
-            else:
-                slab.step(n, mode, stream)
+        def run(n_total):
+            while n_total > 0:
+                t = state["t"]
+                n = min(gblock, n_total, BLOCK - t % BLOCK)
+                if targets is not None and t % BLOCK == 0:
+                    slab["ctrl_pos"].copy_(targets[(t // BLOCK) % len(targets)])
+                if graph is not None and n == gblock:
+                    graph.launch()
+                    state["graph_launches"] += 1
+                else:
+                    slab.step(n, mode, stream)
+                    state["eager_substeps"] += n
+                state["t"] += n
+                n_total -= n
 
-        w, k = warmup, 0
-        while w > 0:
-            n = min(BLOCK, w)
-            do_block(k, n)
-            w -= n
-            k += 1
+        run(warmup)
         stream.synchronize()
         torch.cuda.synchronize()
+        state["graph_launches"] = state["eager_substeps"] = 0
         t0 = time.perf_counter()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        s, k = steps, 0
-        while s > 0:
-            n = min(BLOCK, s)
-            do_block(k, n)
-            s -= n
-            k += 1
+        run(steps)
         ev1.record(stream)
         stream.synchronize()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         gpu_ms = ev0.elapsed_time(ev1)
+        timed = dict(state)
         # per-substep latency: one substep at a time, each bracketed by events on the launch stream
         lat = []
         for _ in range(min(n_lat, steps)):
@@ -132,7 +137,7 @@ def run_path(wl, dm, B, steps, warmup, mode, targets, use_graph, n_lat=200):
             e1.synchronize()
             lat.append(e0.elapsed_time(e1))
         nb = slab["n_broad"].float().mean().item()
-    return wall, gpu_ms, lat, nb, slab
+    return wall, gpu_ms, lat, nb, slab, timed, gblock
 
 
 COPY_BYTES = 1 << 30
@@ -172,30 +177,114 @@ def measured_traffic(kernel, wl, B):
     return fig["traffic"]["bytes_per_launch"], os.path.basename(path)
 
 
-def cpu_baseline(wl, n_envs=128, steps=200):
-    """CPU restatement (FP32, same step) on a bounded sample of the workload: n_envs x steps, at 1 thread and at
-    min(nproc, 16) threads."""
+def cpu_model_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _oracle_init(wl, o, n_envs, block=0):
+    m = wl.model
+    if wl.name == "franka":
+        o.set("qpos", scenes.franka_initial_qpos(m, n_envs).astype(np.float32).astype(np.float64))
+        o.set("ctrl_mode", np.zeros((n_envs, m.n_dofs)))
+    elif wl.name == "quadruped":
+        o.set("qpos", scenes.quadruped_initial_qpos(m, n_envs).astype(np.float32).astype(np.float64))
+        o.set("ctrl_mode", scenes.quadruped_ctrl_mode(m, n_envs))
+    t = wl.targets(n_envs, block)
+    if t is not None:
+        o.set("ctrl_pos", t.astype(np.float32).astype(np.float64))
+    o.forward_kinematics()
+
+
+def cpu_baseline(wl):
+    """The FP32 CPU restatement of the same step, timed on this host at 1 thread and at min(nproc, 16) threads, on
+    a bounded sample: Franka x64 envs x 1000 steps with the targets resampled every 100 steps (the reference's
+    CPU-sized configuration); box pile x32 x 100 steps; quadruped x64 x 500 steps."""
     from tests.oracle_lib import Oracle
 
-    m = wl.model
+    n_envs, steps = {"franka": (64, 1000), "boxpile": (32, 100), "quadruped": (64, 500)}[wl.name]
     out = []
     nthreads = min(os.cpu_count() or 1, 16)
     for threads in (1, nthreads):
-        o = Oracle(m, n_envs, precision=32)
-        if wl.name == "franka":
-            o.set("qpos", scenes.franka_initial_qpos(m, n_envs))
-            o.set("ctrl_mode", np.zeros((n_envs, m.n_dofs)))
-        elif wl.name == "quadruped":
-            o.set("qpos", scenes.quadruped_initial_qpos(m, n_envs))
-            o.set("ctrl_mode", scenes.quadruped_ctrl_mode(m, n_envs))
-        t = wl.targets(n_envs, 0)
-        if t is not None:
-            o.set("ctrl_pos", t)
-        o.forward_kinematics()
+        o = Oracle(wl.model, n_envs, precision=32)
+        _oracle_init(wl, o, n_envs)
+        dt = 0.0
+        for k in range((steps + BLOCK - 1) // BLOCK):
+            t = wl.targets(n_envs, k)
+            if t is not None:
+                o.set("ctrl_pos", t.astype(np.float32).astype(np.float64))
+            n = min(BLOCK, steps - k * BLOCK)
+            t0 = time.perf_counter()
+            o.step(n, threads)
+            dt += time.perf_counter() - t0
+        out.append(n_envs * steps / dt)
+    return out[0], out[1], nthreads, f"{wl.name} x{n_envs} envs x {steps} steps"
+
+
+def drift_vs_fp64(wl, dm):
+    """FP32 device trajectory against the FP64 CPU restatement (and the FP32 restatement against FP64, for scale) on
+    a sample of the workload from the same seeded state, targets held at their first draw: max over dofs of
+    |q32 - q64| per environment, summarised over environments."""
+    from tests.oracle_lib import Oracle
+
+    n_envs, steps = {"franka": (64, 1000), "boxpile": (32, 500), "quadruped": (64, 1000)}[wl.name]
+    slab = make_slab(wl, dm, n_envs)
+    t = wl.targets(n_envs, 0)
+    if t is not None:
+        slab["ctrl_pos"].copy_(torch.as_tensor(t.T, dtype=torch.float32))
+    slab.step(steps, rt.RS_MODE_FUSED)
+    torch.cuda.synchronize()
+    q_dev = slab["qpos"].double().cpu().numpy().T
+    res = {}
+    for p in (64, 32):
+        o = Oracle(wl.model, n_envs, precision=p)
+        _oracle_init(wl, o, n_envs)
+        o.step(steps, min(os.cpu_count() or 1, 16))
+        res[p] = o.get("qpos")
+    d = np.abs(q_dev - res[64]).max(axis=1)
+    d32 = np.abs(res[32] - res[64]).max(axis=1)
+    return {"sample": f"{wl.name} x{n_envs} envs x {steps} steps", "unit": "rad or m (max over qpos entries)",
+            "device_fp32": {"median": float(np.median(d)), "p90": float(np.percentile(d, 90)), "max": float(d.max())},
+            "cpu_fp32": {"median": float(np.median(d32)), "p90": float(np.percentile(d32, 90)),
+                         "max": float(d32.max())}}
+
+
+def like_for_like(n_envs=30000, steps=1000):
+    """The reference's own speed benchmark (examples/speed_benchmark/franka.py: plane + Panda MJCF, dt 0.01,
+    n_envs=30000, PD to the collision-free pose [0,0,0,-1,0,1,0,0.02,0.02] from the neutral pose, 1000 scene.step()
+    calls), run through this library's Scene API (autotuned, graph-replayed fused step). The reference publishes
+    43 M env-steps/s for it on one RTX 4090 (README.md:60, franka.py:38)."""
+    import genesis_amd as gs
+
+    gs.init(backend=gs.gpu)
+    try:
+        scene = gs.Scene(rigid_options=gs.options.RigidOptions(dt=0.01), show_viewer=False)
+        scene.add_entity(gs.morphs.Plane())
+        franka = scene.add_entity(gs.morphs.MJCF(file="xml/franka_emika_panda/panda.xml"))
+        scene.build(n_envs=n_envs)
+        franka.control_dofs_position(torch.tile(torch.tensor([0, 0, 0, -1.0, 0, 1.0, 0, 0.02, 0.02],
+                                                             device="cuda"), (n_envs, 1)))
+        for _ in range(20):  # autotune on the first step, graph capture, warm-up
+            scene.step()
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        o.step(steps, threads)
-        out.append(n_envs * steps / (time.perf_counter() - t0))
-    return out[0], out[1], nthreads
+        for _ in range(steps):
+            scene.step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return {"workload": f"examples/speed_benchmark/franka.py: Franka x{n_envs}, fixed PD target, "
+                            f"{steps} scene.step() calls", "value": n_envs * steps / wall, "unit": "env-steps/s",
+                "ms_per_step": 1e3 * wall / steps, "published": 43e6,
+                "published_hardware": "1x NVIDIA RTX 4090 (README.md:60)",
+                "ratio_to_published": n_envs * steps / wall / 43e6, "fused_config": scene.dmodel.fused_config}
+    finally:
+        gs.destroy()
 
 
 def serving_bench(args):
@@ -255,6 +344,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-drift", action="store_true", help="skip the FP32-vs-FP64 drift sample")
+    ap.add_argument("--like-for-like", action=argparse.BooleanOptionalAction, default=True,
+                    help="franka: also run the reference's speed benchmark (B=30000, fixed target) via the Scene API")
     ap.add_argument("--fused-config", default=None, metavar="EPB,LDS,SCHED,OCC",
                     help="fixed fused launch shape (e.g. 1,1,0,0) instead of autotuning")
     args = ap.parse_args()
@@ -289,16 +381,16 @@ def main():
         tune = rt.autotune_fused(probe, steps=10)
         del probe
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
-    wall_f, gpu_ms_f, lat_f, nb_f, slab = run_path(wl, dm, B, steps, args.warmup, rt.RS_MODE_FUSED, targets,
-                                                   use_graph=True)
+    wall_f, gpu_ms_f, lat_f, nb_f, slab, timed, gblock = run_path(wl, dm, B, steps, args.warmup, rt.RS_MODE_FUSED,
+                                                                  targets, use_graph=True)
     errno = slab.errno()
     print(f"[bench] fused: {wall_f:.3f}s for {steps} steps", file=sys.stderr, flush=True)
     plain = None
     if not args.no_plain:
         # the per-stage path on its own layout ([row][env] planes, coalesced for one lane per environment)
         dm_plain = rt.DeviceModel(model, rt.LAYOUT_PLANES)
-        wall_p, gpu_ms_p, lat_p, nb_p, _ = run_path(wl, dm_plain, B, steps, args.warmup, rt.RS_MODE_PLAIN, targets,
-                                                    use_graph=False, n_lat=50)
+        wall_p, gpu_ms_p, lat_p, nb_p, *_ = run_path(wl, dm_plain, B, steps, args.warmup, rt.RS_MODE_PLAIN, targets,
+                                                     use_graph=False, n_lat=50)
         plain = {"value": B * steps / wall_p, "ms_per_step": 1e3 * wall_p / steps,
                  "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))}
         print(f"[bench] plain: {wall_p:.3f}s", file=sys.stderr, flush=True)
@@ -321,7 +413,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl.desc.format(B=B), "n_envs": B,
-                   "impl": f"fused substep kernel (one wavefront per env), {BLOCK}-substep hipGraph",
+                   "impl": f"fused substep kernel (one wavefront per env), {gblock}-substep hipGraph replayed "
+                           f"{timed['graph_launches']}x in the timed region, {timed['eager_substeps']} eager substeps",
                    "fused_config": dm.fused_config},
         "p50_step_ms": float(np.percentile(lat_f, 50)),
         "p99_step_ms": float(np.percentile(lat_f, 99)),
@@ -342,11 +435,16 @@ def main():
         out["hbm_copy_GBps"] = None
         print(f"[bench] copy peak failed: {e}", file=sys.stderr)
     if not args.no_cpu_baseline and rank == 0:
-        v1, vn, nt = cpu_baseline(wl)
+        v1, vn, nt, sample = cpu_baseline(wl)
         out["cpu_baseline"] = {"value": v1, "unit": "env-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.workload} x128 envs x 200 steps, same seeds, FP32 C++ restatement",
-                               "value_all_cores": vn, "cores_all": nt,
-                               "cpu": platform.processor() or platform.machine()}
+                               "sample": f"{sample}, same seeds, FP32 C++ restatement of the step",
+                               "value_all_cores": vn, "cores_all": nt, "cpu": cpu_model_name()}
+    if not args.no_drift and rank == 0:
+        print("[bench] drift vs the FP64 restatement", file=sys.stderr, flush=True)
+        out["drift_vs_fp64"] = drift_vs_fp64(wl, dm)
+    if args.like_for_like and args.workload == "franka" and rank == 0:
+        print("[bench] like-for-like: examples/speed_benchmark/franka.py", file=sys.stderr, flush=True)
+        out["like_for_like"] = like_for_like()
     if rank == 0:
         print(json.dumps(out), flush=True)
 

@@ -475,6 +475,17 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
     return RS_OK;
 }
 
+int rs_model_update(rs_model* m, const rs_model_header* h, const void* blob, size_t blob_bytes, void* stream) {
+    if (!m || !h || !blob) return fail(RS_E_INVALID, "rs_model_update: null argument");
+    if (blob_bytes != m->blob_bytes || std::memcmp(h, &m->h, sizeof(rs_model_header)) != 0)
+        return fail(RS_E_INVALID, "rs_model_update: the header or blob size differs from the model's (only array "
+                                  "values may change)");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_OK(hipMemcpyAsync(m->blob, blob, blob_bytes, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return RS_OK;
+}
+
 int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
     if (!m || !out) return fail(RS_E_INVALID, "rs_fused_config_get: null argument");
     out->envs_per_block = m->fa.epb;

@@ -373,7 +373,7 @@ def compile_model(entities: list[EntitySpec], options: SolverOptions | None = No
     n_box = int((gtype == GEOM_BOX).sum())
     model.n_contacts_per_pair = 5
     if opt.box_box_detection and n_box > 1:
-        model.n_contacts_per_pair = max(5, 16)
+        raise NotImplementedError("box_box_detection=True: box-box separating-axis detection is not built")
     model.max_collision_pairs = min(opt.max_collision_pairs, model.n_pairs)
     model.max_contact_pairs = model.max_collision_pairs * model.n_contacts_per_pair
     model.max_collision_pairs_broad = model.max_collision_pairs * opt.multiplier_collision_broad_phase
@@ -526,6 +526,21 @@ def _resolve_equalities_at_qpos0(model: CompiledModel, specs):
                 d[6:10] = tf.quat_normalize(tf.quat_mul(tf.quat_inv(lquat[l1]), lquat[l2]))
 
 
+def meaninertia_from_mass(M: np.ndarray, entity_dof_start, entity_dof_end) -> float:
+    """Restates ``kernel_init_meaninertia`` (``abd/misc.py:103-123``): a running sum over entities of the diagonal of M
+    in each entity's dof range, divided by the total dof count after every entity (so an earlier entity's share is
+    divided once per later entity). For a single articulated entity this is trace(M)/D."""
+    D = M.shape[0]
+    if D == 0:
+        return 1.0
+    acc = 0.0
+    for ds, de in zip(entity_dof_start, entity_dof_end):
+        for i_d in range(int(ds), int(de)):
+            acc = acc + float(M[i_d, i_d])
+        acc = acc / D
+    return float(acc)
+
+
 def _init_invweight_and_meaninertia(model: CompiledModel, given_link_invweight):
     """Restates ``rigid_solver.py:492-655``: invweights from J M^-1 J^T at qpos0 (kept when already provided)."""
     A = model.arrays
@@ -565,7 +580,7 @@ def _init_invweight_and_meaninertia(model: CompiledModel, given_link_invweight):
                 dof_iw[ds : ds + 3] = Ad.mean()
             else:
                 dof_iw[ds] = Ad[0]
-        meaninertia = float(np.trace(M) / D)
+        meaninertia = meaninertia_from_mass(M, A["entity_dof_start"], A["entity_dof_end"])
         model.arrays["_M0"] = M
     A["link_invweight"] = link_iw
     A["dof_invweight"] = dof_iw

@@ -24,10 +24,12 @@ RS_MODE_PLAIN, RS_MODE_FUSED = 0, 1
 STAGES = ["fk", "crba", "factor", "rne", "aabb", "broadphase", "narrowphase", "constraints", "solve", "integrate"]
 STAGE_ID = {n: i for i, n in enumerate(STAGES)}
 
+# per-environment error bits, the values of the reference's ErrorCode (genesis/utils/array_class.py:91-97)
 ERR_OVERFLOW_CANDIDATE_CONTACTS = 1 << 0
 ERR_OVERFLOW_COLLISION_PAIRS = 1 << 1
-ERR_INVALID_FORCE_NAN = 1 << 2
-ERR_INVALID_ACC_NAN = 1 << 3
+ERR_OVERFLOW_HIBERNATION_ISLANDS = 1 << 2
+ERR_INVALID_FORCE_NAN = 1 << 3
+ERR_INVALID_ACC_NAN = 1 << 4
 
 COUNT_KINDS = [
     "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
@@ -69,6 +71,8 @@ _SIGNATURES = {
     "rs_model_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "rs_model_destroy": (None, [ctypes.c_void_p]),
+    "rs_model_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_void_p]),
     "rs_state_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_int64)]),
     "rs_state_field_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
@@ -168,6 +172,13 @@ class DeviceModel:
             _lib.rs_model_destroy(self.handle)
             self.handle = None
 
+    def update(self, stream=None):
+        """Upload the compiled model's current array values in place (same counts and options): slabs, graphs and
+        the launch configuration of this model stay valid."""
+        header, blob = pack_model(self.model, np.float32)
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        _check(_lib.rs_model_update(self.handle, ctypes.byref(header), buf, len(blob), _stream_handle(stream)))
+
     def field_rows(self, index: int) -> int:
         rows = ctypes.c_int64()
         _check(_lib.rs_state_field_rows(self.handle, index, ctypes.byref(rows)))
@@ -252,14 +263,16 @@ class StepGraph:
 
     def __init__(self, slab: StateSlab, n_substeps: int, mode: int, stream: torch.cuda.Stream):
         self.slab = slab
+        self.dmodel = slab.dmodel  # the graph's kernels read this model's device arrays
         self.stream = stream
         h = ctypes.c_void_p()
         _check(_lib.rs_graph_create(slab.dmodel.handle, slab.ptr, slab.n_envs, n_substeps, mode,
                                     _stream_handle(stream), ctypes.byref(h)))
         self.handle = h
 
-    def launch(self):
-        _check(_lib.rs_graph_launch(self.handle, _stream_handle(self.stream)))
+    def launch(self, stream=None):
+        """Replay on ``stream`` (default: the capture stream)."""
+        _check(_lib.rs_graph_launch(self.handle, _stream_handle(stream or self.stream)))
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
@@ -290,25 +303,31 @@ class SharedLaunch:
             self.handle = None
 
 
-def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=None) -> dict:
-    """Pick the fused kernel's launch shape (environments per workgroup, LDS or work-field private regions) for this
-    model and batch by timing ``steps`` substeps of each candidate on a copy of the slab; the slab itself is left
-    unchanged. The winner is stored in the model and returned with the timings (ms per substep)."""
+def _config_name(c):
+    return "epb{}_{}{}{}".format(c[0], "lds" if c[1] else "hbm", "" if c[2] else "_nosched",
+                                 f"_occ{c[3]}" if c[3] else "")
+
+
+def autotune_fused(slab: StateSlab, steps: int = 10, candidates=None, stream=None, repeats: int = 3,
+                   finalists: int = 4) -> dict:
+    """Pick the fused kernel's launch shape (environments per workgroup, LDS or work-field private regions,
+    cost-ordered scheduling, register budget) for this model and batch. Every runnable candidate is timed once over
+    ``steps`` substeps on a copy of the slab (the slab itself is left unchanged); the ``finalists`` fastest are timed
+    ``repeats`` more times each, and the one with the lowest median wins. The winner is stored in the model and
+    returned with the timings (ms per substep, the median for finalists)."""
     dm = slab.dmodel
     if candidates is None:
         candidates = [(e, lds, sch, occ) for occ in (0, 3, 4) for sch in (True, False) for lds in (True, False)
                       for e in (1, 2, 4, 8)]
+    candidates = [tuple(c) + (True, 0)[len(c) - 2:] if len(c) < 4 else tuple(c) for c in candidates]
     saved = dm.fused_config
     scratch = StateSlab.__new__(StateSlab)
     scratch.__dict__.update(slab.__dict__)
     scratch.buffer = slab.buffer.clone()
     s = stream or torch.cuda.current_stream()
-    timings = {}
-    for cand in candidates:
-        try:
-            dm.set_fused_config(*cand)
-        except RigidStepError:
-            continue
+
+    def time_one(cand):
+        dm.set_fused_config(*cand)
         scratch.buffer.copy_(slab.buffer)
         scratch.step(2, RS_MODE_FUSED, s)  # warm-up
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -316,17 +335,72 @@ def autotune_fused(slab: StateSlab, steps: int = 20, candidates=None, stream=Non
         scratch.step(steps, RS_MODE_FUSED, s)
         e1.record(s)
         e1.synchronize()
-        timings[tuple(cand)] = e0.elapsed_time(e1) / steps
+        return e0.elapsed_time(e1) / steps
+
+    timings = {}
+    for cand in candidates:
+        try:
+            timings[cand] = [time_one(cand)]
+        except RigidStepError:
+            continue
+    for cand in sorted(timings, key=lambda c: timings[c][0])[:finalists]:
+        timings[cand] += [time_one(cand) for _ in range(repeats)]
     del scratch
     if not timings:
         dm.set_fused_config(saved["envs_per_block"], saved["use_lds"], saved["schedule"], saved["occupancy"])
         raise RigidStepError("autotune: no runnable fused configuration")
-    best = min(timings, key=timings.get)
+    med = {c: float(np.median(t)) for c, t in timings.items()}
+    best = min((c for c in timings if len(timings[c]) > 1), key=med.get, default=min(med, key=med.get))
     dm.set_fused_config(*best)
-    def name(c):
-        return "epb{}_{}{}{}".format(c[0], "lds" if c[1] else "hbm", "" if len(c) < 3 or c[2] else "_nosched",
-                                     f"_occ{c[3]}" if len(c) > 3 and c[3] else "")
+    return {"envs_per_block": best[0], "use_lds": bool(best[1]), "schedule": bool(best[2]), "occupancy": best[3],
+            "ms_per_substep": med[best], "timings": {_config_name(c): t for c, t in med.items()}}
 
-    return {"envs_per_block": best[0], "use_lds": best[1], "schedule": best[2] if len(best) > 2 else True,
-            "occupancy": best[3] if len(best) > 3 else 0, "ms_per_substep": timings[best],
-            "timings": {name(c): t for c, t in timings.items()}}
+
+def _file_digest(path: str) -> str:
+    import hashlib
+
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()[:16]
+
+
+def autotune_cache_path() -> str:
+    return os.path.join(os.environ.get("GENESIS_AMD_CACHE", os.path.join(os.path.expanduser("~"), ".cache",
+                                                                         "genesis_amd")), "autotune.json")
+
+
+def autotune_fused_cached(slab: StateSlab, **kw) -> dict:
+    """``autotune_fused`` with its result kept on disk, keyed by the model blob, the batch size, the device name and
+    the library build: a later process with the same key reuses the launch shape without timing."""
+    import hashlib
+    import json
+
+    header, blob = pack_model(slab.dmodel.model, np.float32)
+    key = "|".join([hashlib.sha256(bytes(header) + blob).hexdigest()[:16], str(slab.n_envs),
+                    torch.cuda.get_device_name(), _file_digest(LIB_PATH)])
+    path = autotune_cache_path()
+    try:
+        with open(path) as f:
+            cache = json.load(f)
+    except (OSError, ValueError):
+        cache = {}
+    hit = cache.get(key)
+    if hit is not None:
+        try:
+            slab.dmodel.set_fused_config(hit["envs_per_block"], hit["use_lds"], hit["schedule"], hit["occupancy"])
+            return dict(hit, cached=True)
+        except RigidStepError:
+            pass
+    res = autotune_fused(slab, **kw)
+    cache[key] = {k: res[k] for k in ("envs_per_block", "use_lds", "schedule", "occupancy", "ms_per_substep")}
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return dict(res, cached=False)

@@ -90,7 +90,11 @@ class Scene:
             enable_multi_contact=ro.enable_multi_contact, constraint_solver=int(ro.constraint_solver),
             integrator=int(ro.integrator))
 
-    def build(self, n_envs: int = 0, env_spacing=(0.0, 0.0), compile_kernels=True, **_ignored):
+    def build(self, n_envs: int = 0, env_spacing=(0.0, 0.0), compile_kernels=True, autotune: bool = True,
+              **_ignored):
+        """Compile the model and allocate the state (``scene.py:841-905``). ``autotune`` (fused step only): choose
+        the fused kernel's launch shape by timing on the first ``step()`` (the state is the user's by then), or reuse
+        the shape kept on disk for this model, batch size, device and library build."""
         if self.is_built:
             raise GenesisException("Scene is already built")
         from . import _global_state
@@ -122,18 +126,41 @@ class Scene:
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.dmodel = runtime.DeviceModel(self.model)
         self.slab = runtime.StateSlab(self.dmodel, self._B, self.device)
+        self._autotune = bool(autotune)
+        self._graph = None
+        self.autotune_result = None
         self.is_built = True
         torch.cuda.synchronize()
         return self
 
     # ------------------------------------------------------------------ stepping
+    def _fused_graph(self):
+        """The hipGraph of one ``step()`` (``substeps`` fused substeps), captured once on a side stream after the
+        launch shape is chosen, and replayed on the caller's stream, in order with the state writes before it."""
+        if self._graph is None:
+            if self._autotune:
+                self.autotune_result = runtime.autotune_fused_cached(self.slab)
+            torch.cuda.synchronize()
+            self._capture_stream = torch.cuda.Stream()
+            self._graph = runtime.StepGraph(self.slab, self.substeps, runtime.RS_MODE_FUSED, self._capture_stream)
+            torch.cuda.synchronize()
+        return self._graph
+
     def step(self, update_visualizer=True, refresh_visualizer=True):
-        mode = runtime.RS_MODE_FUSED if self.step_impl == "fused" else runtime.RS_MODE_PLAIN
-        for _ in range(self.substeps):
-            self.slab.step(1, mode)
-            self._substep_count += 1
-            if self._substep_count % RATE_CHECK_ERRNO == 0:
+        """``scene.step`` (``scene.py:1017-1037``): ``substeps`` rigid substeps; the error bits are polled every
+        ``RATE_CHECK_ERRNO`` substeps (``simulator.py:272-296``)."""
+        if self.step_impl == "fused":
+            self._fused_graph().launch(torch.cuda.current_stream())
+            before = self._substep_count
+            self._substep_count += self.substeps
+            if self._substep_count // RATE_CHECK_ERRNO != before // RATE_CHECK_ERRNO:
                 self.check_errno()
+        else:
+            for _ in range(self.substeps):
+                self.slab.step(1, runtime.RS_MODE_PLAIN)
+                self._substep_count += 1
+                if self._substep_count % RATE_CHECK_ERRNO == 0:
+                    self.check_errno()
         self.t += 1
 
     def check_errno(self):
@@ -150,6 +177,8 @@ class Scene:
             raise GenesisException("Invalid constraint forces causing 'nan'. Please decrease Rigid simulation timestep.")
         if e & runtime.ERR_INVALID_ACC_NAN:
             raise GenesisException("Invalid accelerations causing 'nan'. Please decrease Rigid simulation timestep.")
+        if e & runtime.ERR_OVERFLOW_HIBERNATION_ISLANDS:
+            raise GenesisException("Contact island buffer overflow. Please increase RigidOptions 'max_collision_pairs'.")
 
     def get_error_envs_mask(self):
         return self.slab["errbits"][0] > 0
@@ -349,29 +378,67 @@ class Scene:
             A["dof_force_range"][d] = v
         elif kind in ("damping", "stiffness"):
             A[f"dof_{kind}"][d] = np.broadcast_to(v, (len(d),))
-        torch.cuda.synchronize()
-        self.dmodel = runtime.DeviceModel(self.model)
-        self.slab.dmodel = self.dmodel
+        self.dmodel.update()
 
     # ------------------------------------------------------------------ contacts
-    def get_contacts(self, entity=None, with_entity=None, exclude_self_contact=False):
-        """Contact arrays of every environment, padded to the largest count (collider.py:810-913)."""
-        n = self.slab["n_contacts"][0]
-        kmax = int(n.max().item()) if n.numel() else 0
+    def _contact_arrays(self):
+        """Every environment's contacts padded to the largest count, batch first (``Collider.get_contacts``,
+        ``collider.py:810-913``, as_tensor): integer entries beyond an environment's count are -1 and float entries
+        0, as the reference's copying path fills them."""
+        B = self._B
         s = self.slab
-        out = {
-            "link_a": s["contact_link_a"][:kmax].t().clone(),
-            "link_b": s["contact_link_b"][:kmax].t().clone(),
-            "geom_a": s["contact_geom_a"][:kmax].t().clone(),
-            "geom_b": s["contact_geom_b"][:kmax].t().clone(),
-            "penetration": s["contact_penetration"][:kmax].t().clone(),
-            "position": s["contact_pos"].view(-1, 3, self._B)[:kmax].permute(2, 0, 1).clone(),
-            "normal": s["contact_normal"].view(-1, 3, self._B)[:kmax].permute(2, 0, 1).clone(),
-            "force": s["contact_force"].view(-1, 3, self._B)[:kmax].permute(2, 0, 1).clone(),
+        n = s["n_contacts"][0].long()
+        kmax = int(n.max().item()) if n.numel() else 0
+        valid = torch.arange(kmax, device=self.device)[None, :] < n[:, None]  # [B, kmax]
+
+        def ints(name):
+            return torch.where(valid, s[name][:kmax].t(), torch.full_like(valid, -1, dtype=torch.int32))
+
+        def floats(name, width):
+            v = s[name][:kmax].t() if width == 1 else s[name].view(-1, width, B)[:kmax].permute(2, 0, 1)
+            m = valid if width == 1 else valid[..., None]
+            return torch.where(m, v, torch.zeros_like(v))
+
+        return {
+            "link_a": ints("contact_link_a"), "link_b": ints("contact_link_b"),
+            "geom_a": ints("contact_geom_a"), "geom_b": ints("contact_geom_b"),
+            "penetration": floats("contact_penetration", 1), "position": floats("contact_pos", 3),
+            "normal": floats("contact_normal", 3), "force": floats("contact_force", 3),
         }
+
+    def get_contacts(self, entity=None, with_entity=None, exclude_self_contact=False):
+        """Contacts of the last step. Without ``entity``: every contact (``Collider.get_contacts``). With it:
+        ``RigidEntity.get_contacts`` (``rigid_entity.py:3918-3998``): the contacts that involve the entity (only those
+        with ``with_entity`` when given; only its self-contacts when ``with_entity`` is the entity itself; no
+        self-contacts with ``exclude_self_contact``), ``force`` replaced by ``force_a = -force`` and
+        ``force_b = +force``; batched scenes keep every row and add ``valid_mask``, unbatched ones keep only the
+        selected rows."""
+        data = self._contact_arrays()
+        if entity is None:
+            if self.n_envs == 0:
+                data = {k: v[0] for k, v in data.items()}
+            return data
+        ga, gb = data["geom_a"], data["geom_b"]
+
+        def inside(g, ent):
+            return (g >= ent.geom_start) & (g < ent.geom_end)
+
+        op = torch.logical_xor if exclude_self_contact else torch.logical_or
+        if with_entity is not None and entity.idx == with_entity.idx:
+            if exclude_self_contact:
+                raise GenesisException("`with_entity` is self but `exclude_self_contact` is True.")
+            op = torch.logical_and
+        valid = op(inside(ga, entity), inside(gb, entity))
+        if with_entity is not None and entity.idx != with_entity.idx:
+            valid = valid & (inside(ga, with_entity) | inside(gb, with_entity))
         if self.n_envs == 0:
-            out = {k: v[0] for k, v in out.items()}
-        return out
+            data = {k: v[0][valid[0]] for k, v in data.items()}
+        else:
+            data["valid_mask"] = valid
+        force = data.pop("force")
+        data["force_a"] = -force
+        data["force_b"] = force
+        return data
 
 
 def _xyz_to_quat(e):

@@ -38,13 +38,14 @@ enum {
     RS_E_UNSUPPORTED = -4,
 };
 
-/* Per-environment error bits; values equal the reference ErrorCode (utils/array_class.py:91-101). */
+/* Per-environment error bits; values equal the reference ErrorCode (utils/array_class.py:91-97). Hibernation is not
+ * built, so OVERFLOW_HIBERNATION_ISLANDS is never set; its value is reserved so that the other bits keep theirs. */
 enum {
     RS_ERR_OVERFLOW_CANDIDATE_CONTACTS = 1 << 0,
     RS_ERR_OVERFLOW_COLLISION_PAIRS = 1 << 1,
-    RS_ERR_INVALID_FORCE_NAN = 1 << 2,
-    RS_ERR_INVALID_ACC_NAN = 1 << 3,
-    RS_ERR_OVERFLOW_HIBERNATION_ISLANDS = 1 << 4,
+    RS_ERR_OVERFLOW_HIBERNATION_ISLANDS = 1 << 2,
+    RS_ERR_INVALID_FORCE_NAN = 1 << 3,
+    RS_ERR_INVALID_ACC_NAN = 1 << 4,
 };
 
 /* Step implementations. */
@@ -129,6 +130,12 @@ typedef struct rs_model rs_model;
 /* Upload a model blob (layout per the directory above) to the current device. */
 int rs_model_create(const rs_model_header* header, const void* blob, size_t blob_bytes, rs_model** out);
 void rs_model_destroy(rs_model* model);
+
+/* Overwrite the model's array values with a blob of the same header (counts, options and offsets unchanged), in
+ * stream order, then synchronise the stream; the device address, the launch configuration and captured graphs stay
+ * valid and see the new values. Used by the dof parameter setters (set_dofs_kp / kv / force_range / damping /
+ * stiffness, engine/entities/rigid_entity/rigid_entity.py:3372-3500, which write dofs_info in place). */
+int rs_model_update(rs_model* model, const rs_model_header* header, const void* blob, size_t blob_bytes, void* stream);
 
 /* ------------------------------------------------------------------------------------------------ state slab */
 

@@ -52,3 +52,39 @@ def test_pack_model_offsets_are_aligned_and_in_bounds():
     for i in range(h.n_arrays):
         assert h.array_offset[i] % 64 == 0
         assert h.array_offset[i] <= len(blob)
+
+
+# ErrorCode of the reference (genesis/utils/array_class.py:91-97), written out here as numbers
+REFERENCE_ERROR_CODE = {
+    "OVERFLOW_CANDIDATE_CONTACTS": 0b1,
+    "OVERFLOW_COLLISION_PAIRS": 0b10,
+    "OVERFLOW_HIBERNATION_ISLANDS": 0b100,
+    "INVALID_FORCE_NAN": 0b1000,
+    "INVALID_ACC_NAN": 0b10000,
+}
+
+
+def _header_enum_values():
+    text = open(os.path.join(REPO, "include", "rigidstep.h")).read()
+    return {m.group(1): 1 << int(m.group(2)) for m in re.finditer(r"RS_ERR_([A-Z_]+)\s*=\s*1\s*<<\s*(\d+)", text)}
+
+
+def test_error_bits_equal_reference_error_code():
+    header = _header_enum_values()
+    assert header == REFERENCE_ERROR_CODE
+    for name, value in REFERENCE_ERROR_CODE.items():
+        assert getattr(runtime, "ERR_" + name) == value, name
+
+
+def test_unbuilt_options_are_rejected():
+    import pytest
+
+    from genesis_amd.options import RigidOptions
+
+    for kw in (dict(box_box_detection=True), dict(prefer_parallel_linesearch=True), dict(broadphase_traversal=1),
+               dict(use_gjk_collision=True)):
+        with pytest.raises(NotImplementedError):
+            RigidOptions(**kw).validate()
+    for kw in (dict(prefer_parallel_linesearch=None), dict(prefer_parallel_linesearch=False),
+               dict(broadphase_traversal=None), dict(broadphase_traversal=0)):
+        RigidOptions(**kw).validate()

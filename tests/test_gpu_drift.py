@@ -2,8 +2,11 @@
 precision='64'), over fixed horizons, for every workload.
 
 Stated tolerances (set from the FP32 CPU restatement's own drift against FP64, measured on the same seeds):
-* Franka x64, 1000 steps: median over envs of max |q32 - q64| <= 1e-4 rad, 90th percentile <= 5e-3 rad (a few envs
-  in chaotic multi-contact states diverge in both FP32 implementations alike);
+* Franka x64, 1000 steps: median over envs of max |q32 - q64| <= 1e-4 rad, 90th percentile <= 5e-3 rad; the worst
+  env within twice the FP32 CPU restatement's own worst drift (and within 1e-2 rad if that is smaller), and at most
+  twice as many envs as the FP32 restatement (plus one) beyond 1e-2 rad. A few envs start in deep self-penetration and
+  move chaotically: the FP32 restatement itself ends up to ~0.5 rad from FP64 on one or two of the 64, so a flat 1e-2
+  rad bound on the worst env would fail for any FP32 implementation;
 * quadruped x64, 1000 steps: base position/orientation drift <= 1e-4 for every env;
 * box pile x32: per-box position drift <= 1e-4 m over the first 50 steps; after 500 steps (settled, chaotic order of
   contacts) the mean resting height agrees to 1e-3 m.
@@ -57,13 +60,23 @@ def test_franka_drift_vs_fp64():
         o.set("ctrl_pos", t0.astype(np.float32).astype(np.float64))
 
     rt, slab, o = _pair(m, B, init_slab, init_oracle)
+    o32 = Oracle(m, B, precision=32)
+    init_oracle(o32)
+    o32.forward_kinematics()
     slab.step(1000, rt.RS_MODE_FUSED)
     torch.cuda.synchronize()
     o.step(1000, 16)
-    d = np.abs(get_dev(slab, "qpos") - o.get("qpos")).max(axis=1)
-    print(f"[drift] franka 1000 steps: median {np.median(d):.2e} p90 {np.percentile(d, 90):.2e} max {d.max():.2e}")
+    o32.step(1000, 16)
+    q64 = o.get("qpos")
+    d = np.abs(get_dev(slab, "qpos") - q64).max(axis=1)
+    d32 = np.abs(o32.get("qpos") - q64).max(axis=1)
+    print(f"[drift] franka 1000 steps: median {np.median(d):.2e} p90 {np.percentile(d, 90):.2e} max {d.max():.2e}; "
+          f"FP32 restatement: median {np.median(d32):.2e} max {d32.max():.2e}, "
+          f"envs > 1e-2 rad: device {(d > 1e-2).sum()} restatement {(d32 > 1e-2).sum()}", flush=True)
     assert np.median(d) <= 1e-4
     assert np.percentile(d, 90) <= 5e-3
+    assert d.max() <= 2.0 * max(d32.max(), 1e-2)
+    assert (d > 1e-2).sum() <= 2 * (d32 > 1e-2).sum() + 1
 
 
 def test_quadruped_drift_vs_fp64():

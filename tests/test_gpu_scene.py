@@ -86,12 +86,43 @@ def test_contacts_and_errno(gs):
     c = scene.get_contacts()
     assert c["position"].shape[0] == B and c["position"].shape[-1] == 3
     assert (scene.slab["n_contacts"][0] > 0).any()
+    _check_entity_contacts(scene, franka, c)
     # too few contact slots: the step sets the overflow bit and the poll raises, as Genesis does
     scene2, franka2 = _franka_scene(gs, B, max_collision_pairs=1)
     franka2.set_dofs_position(q)
     with pytest.raises(gs.GenesisException):
         for _ in range(60):
             scene2.step()
+
+
+def _check_entity_contacts(scene, franka, c):
+    """RigidEntity.get_contacts (rigid_entity.py:3918-3998): valid_mask selects the contacts that involve the entity
+    (with with_entity: also the other entity; with with_entity=self: both geoms in the entity; with
+    exclude_self_contact: exactly one geom in it), force_a = -force and force_b = force, and entries beyond an
+    environment's contact count are padded with -1 / 0."""
+    plane = scene.entities[0]
+    n = scene.slab["n_contacts"][0].long()
+    k = c["geom_a"].shape[1]
+    pad = torch.arange(k, device=n.device)[None] >= n[:, None]
+    assert bool((c["geom_a"][pad] == -1).all()) and bool((c["position"][pad] == 0).all())
+    ga, gb = c["geom_a"], c["geom_b"]
+    in_f = lambda g: (g >= franka.geom_start) & (g < franka.geom_end)  # noqa: E731
+    in_p = lambda g: (g >= plane.geom_start) & (g < plane.geom_end)  # noqa: E731
+    cases = [
+        (dict(), in_f(ga) | in_f(gb)),
+        (dict(with_entity=plane), (in_f(ga) | in_f(gb)) & (in_p(ga) | in_p(gb))),
+        (dict(with_entity=franka), in_f(ga) & in_f(gb)),
+        (dict(exclude_self_contact=True), in_f(ga) ^ in_f(gb)),
+    ]
+    for kw, want in cases:
+        e = franka.get_contacts(**kw)
+        assert set(e) == {"link_a", "link_b", "geom_a", "geom_b", "penetration", "position", "normal", "force_a",
+                          "force_b", "valid_mask"}, kw
+        assert torch.equal(e["valid_mask"], want & ~pad), kw
+        assert torch.equal(e["force_a"], -c["force"]) and torch.equal(e["force_b"], c["force"])
+    assert bool((franka.get_contacts(with_entity=plane)["valid_mask"]).any())
+    with pytest.raises(Exception, match="exclude_self_contact"):
+        franka.get_contacts(with_entity=franka, exclude_self_contact=True)
 
 
 def _stepped_franka(gs, B, n):
