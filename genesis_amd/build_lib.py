@@ -25,16 +25,39 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    src = os.path.join(CSRC, "rs_lib.hip")
+# translation units: the C ABI with the per-stage kernels, and one unit per fused-kernel class (compiled in parallel)
+SOURCES = ["rs_lib.hip", "rs_fused_t16.hip", "rs_fused_t24.hip", "rs_fused_gen.hip", "rs_fused_multi.hip"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every translation unit (always, even when the library exists) and link librigidstep.so in place."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = ["hipcc", *compile_flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, r
+
+    jobs = jobs or min(len(SOURCES), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(compile_one, SOURCES))
+    for src, _, r in results:
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {src} ({r.returncode})")
+        if verbose:
+            sys.stderr.write(r.stderr)
     tmp = OUT + ".tmp"
-    cmd = ["hipcc", *FLAGS, "-o", tmp, src]
+    cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[obj for _, obj, _ in results]]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
-    if verbose:
-        sys.stderr.write(r.stderr)
+        raise RuntimeError(f"hipcc link failed ({r.returncode})")
     os.replace(tmp, OUT)
     return OUT
 

@@ -351,7 +351,7 @@ __device__ __forceinline__ int xcd_block(int blk, int nblk) {
 }
 
 // flips the schedule parity and empties the queues the coming substep appends to (one thread, before each substep)
-__global__ void k_sched_flip(DState st) {
+static __global__ void k_sched_flip(DState st) {
     int* ctl = st.sched + (size_t)2 * kSchedClasses * st.B;
     const int par = (ctl[0] & 1) ^ 1;
     ctl[0] = par;

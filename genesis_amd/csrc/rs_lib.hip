@@ -15,6 +15,7 @@
 #include "rs_constraint.h"
 #include "rs_dyn.h"
 #include "rs_fused.h"
+#include "rs_launch.h"
 
 using namespace rs;
 
@@ -99,6 +100,7 @@ static int64_t count_of(const rs_model_header& h, int kind) {
         case RS_N_GEOMS2: return (int64_t)h.n_geoms * h.n_geoms;
         case RS_N_PAIRS: return h.n_pairs;
         case RS_N_SUPPORT: return h.n_support;
+        case RS_N_VERTS: return h.n_verts;
         case RS_N_CONTACTS: return h.max_contact_pairs;
         case RS_N_CONSTRAINTS: return h.len_constraints;
         case RS_N_CONSTRAINTS_DOFS: return (int64_t)h.len_constraints * h.n_dofs;
@@ -335,29 +337,12 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     set_lds_slice(m->h, &fa, m->use_lds, lds_slice_floats(m->h, fa, m->use_lds));
     const size_t lds = (size_t)epb * fa.lds_wave * 4;
     // one kernel variant per (private region, occupancy) for models with a row tile; models without one run the
-    // generic-solver variants at the compiler's register budget
-    if (fa.tiled == kTileMaxDofs) {
-        if (m->use_lds) {
-            if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<true, 4, 1>), grid, block, lds, s, m->dm, st, fa);
-            else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3, 1>), grid, block, lds, s, m->dm, st, fa);
-            else hipLaunchKernelGGL((k_fused_wave<true, 0, 1>), grid, block, lds, s, m->dm, st, fa);
-        } else {
-            if (m->occupancy == 4) hipLaunchKernelGGL((k_fused_wave<false, 4, 1>), grid, block, lds, s, m->dm, st, fa);
-            else if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3, 1>), grid, block, lds, s, m->dm, st, fa);
-            else hipLaunchKernelGGL((k_fused_wave<false, 0, 1>), grid, block, lds, s, m->dm, st, fa);
-        }
-    } else if (fa.tiled == kTileWideDofs) {
-        if (m->use_lds) {
-            if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<true, 3, 4>), grid, block, lds, s, m->dm, st, fa);
-            else hipLaunchKernelGGL((k_fused_wave<true, 0, 4>), grid, block, lds, s, m->dm, st, fa);
-        } else {
-            if (m->occupancy == 3) hipLaunchKernelGGL((k_fused_wave<false, 3, 4>), grid, block, lds, s, m->dm, st, fa);
-            else hipLaunchKernelGGL((k_fused_wave<false, 0, 4>), grid, block, lds, s, m->dm, st, fa);
-        }
-    } else {
-        if (m->use_lds) hipLaunchKernelGGL((k_fused_wave<true, 0, 0>), grid, block, lds, s, m->dm, st, fa);
-        else hipLaunchKernelGGL((k_fused_wave<false, 0, 0>), grid, block, lds, s, m->dm, st, fa);
-    }
+    // generic-solver variants at the compiler's register budget (each class is instantiated in its own translation
+    // unit: rs_fused_t16.hip, rs_fused_t24.hip, rs_fused_gen.hip)
+    const bool lds_on = m->use_lds != 0;
+    if (fa.tiled == kTileMaxDofs) launch_fused_tile16(lds_on, m->occupancy, grid, block, lds, s, m->dm, st, fa);
+    else if (fa.tiled == kTileWideDofs) launch_fused_tile24(lds_on, m->occupancy, grid, block, lds, s, m->dm, st, fa);
+    else launch_fused_generic(lds_on, grid, block, lds, s, m->dm, st, fa);
     HIP_OK(hipGetLastError());
     return RS_OK;
 }
@@ -705,10 +690,7 @@ int rs_batch_step(rs_batch* bt, int n, const rs_model* const* models, void* cons
     HIP_OK(hipEventRecord(bt->done[k], s));
     bt->pending[k] = true;
     const dim3 grid((unsigned)blocks), block((unsigned)(kWave * epb));
-    if (use_lds)
-        hipLaunchKernelGGL(k_fused_multi<true>, grid, block, (size_t)epb * slice * 4, s, bt->dev[k], n);
-    else
-        hipLaunchKernelGGL(k_fused_multi<false>, grid, block, (size_t)epb * slice * 4, s, bt->dev[k], n);
+    launch_fused_multi(use_lds, grid, block, (size_t)epb * slice * 4, s, bt->dev[k], n);
     HIP_OK(hipGetLastError());
     return RS_OK;
 }

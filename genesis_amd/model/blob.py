@@ -13,7 +13,7 @@ import numpy as np
 
 from .compiler import CompiledModel
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_MODEL_ARRAYS = 96
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _INCLUDE = os.path.normpath(os.path.join(_HERE, "..", "..", "include"))
@@ -21,7 +21,8 @@ _INCLUDE = os.path.normpath(os.path.join(_HERE, "..", "..", "include"))
 COUNT_KINDS = [
     "RS_N_ONE", "RS_N_LINKS", "RS_N_JOINTS", "RS_N_DOFS", "RS_N_QS", "RS_N_GEOMS", "RS_N_ENTITIES",
     "RS_N_EQUALITIES", "RS_N_DOFS2", "RS_N_GEOMS2", "RS_N_PAIRS", "RS_N_SUPPORT", "RS_N_CONTACTS",
-    "RS_N_CONSTRAINTS", "RS_N_CONSTRAINTS_DOFS", "RS_N_BROAD", "RS_N_SORT",
+    "RS_N_CONSTRAINTS", "RS_N_CONSTRAINTS_DOFS", "RS_N_BROAD", "RS_N_SORT", "RS_N_WORK", "RS_N_SCHED",
+    "RS_N_DOFS_PLANE", "RS_N_ONE_PLANE", "RS_N_VERTS",
 ]
 
 
@@ -50,6 +51,7 @@ class ModelHeader(ctypes.Structure):
         ("n_equalities", ctypes.c_int32),
         ("n_pairs", ctypes.c_int32),
         ("n_support", ctypes.c_int32),
+        ("n_verts", ctypes.c_int32),
         ("max_collision_pairs", ctypes.c_int32),
         ("max_contact_pairs", ctypes.c_int32),
         ("max_collision_pairs_broad", ctypes.c_int32),
@@ -91,6 +93,7 @@ def _count(model: CompiledModel, kind: str) -> int:
         "RS_N_GEOMS2": model.n_geoms * model.n_geoms,
         "RS_N_PAIRS": model.n_pairs,
         "RS_N_SUPPORT": len(model.arrays["support_v"]),
+        "RS_N_VERTS": len(model.arrays["verts"]),
     }[kind]
 
 
@@ -108,6 +111,7 @@ def pack_model(model: CompiledModel, float_dtype=np.float32) -> tuple[ModelHeade
     h.n_geoms, h.n_entities, h.n_equalities = model.n_geoms, model.n_entities, model.n_equalities
     h.n_pairs = model.n_pairs
     h.n_support = len(model.arrays["support_v"])
+    h.n_verts = len(model.arrays["verts"])
     h.max_collision_pairs = model.max_collision_pairs
     h.max_contact_pairs = model.max_contact_pairs
     h.max_collision_pairs_broad = model.max_collision_pairs_broad

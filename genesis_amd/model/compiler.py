@@ -686,6 +686,7 @@ def _build_support_tables(model, gverts):
     G = model.n_geoms
     start = np.full(G, -1, np.int64)
     tables = []
+    vids = []
     n = 0
     dirs = direction_grid()
     for i_g in range(G):
@@ -694,8 +695,18 @@ def _build_support_tables(model, gverts):
         v = gverts[i_g].astype(np.float32).astype(np.float64)
         idx = np.argmax(dirs @ v.T, axis=1)
         tables.append(v[idx])
+        vids.append(idx)
         start[i_g] = n
         n += len(idx)
     A["support_cell_start"] = start
     A["support_v"] = np.concatenate(tables, axis=0) if tables else np.zeros((1, 3))
+    # vertex index within its geom (support_field.py:56-66: argmax over the geom's own vertex list)
+    A["support_vid"] = np.concatenate(vids) if vids else np.zeros(1, np.int64)
     A["geom_verts"] = gverts
+    # every geom's vertices back to back (verts_info.init_pos with geoms_info.vert_start / vert_end)
+    counts = np.array([len(v) for v in gverts], np.int64)
+    A["geom_vert_end"] = np.cumsum(counts)
+    A["geom_vert_start"] = A["geom_vert_end"] - counts
+    A["verts"] = np.concatenate(gverts, axis=0) if G else np.zeros((1, 3))
+    if len(A["verts"]) == 0:
+        A["verts"] = np.zeros((1, 3))

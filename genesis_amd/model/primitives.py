@@ -103,3 +103,19 @@ def capsule_entity(radius, length, pos=(0, 0, 0), quat=(1, 0, 0, 0), rho=1000.0,
     izz = 0.5 * m_cyl * radius**2 + 0.4 * m_sph * radius**2
     ixx = m_cyl * (3 * radius**2 + length**2) / 12 + m_sph * (0.4 * radius**2 + 0.25 * length**2 + 3 / 8 * radius * length)
     return _free_body(name, pos, quat, m, np.diag([ixx, ixx, izz]), geom, fixed)
+
+
+def hull_entity(points, pos=(0, 0, 0), quat=(1, 0, 0, 0), rho=1000.0, friction=1.0, fixed=False, name="hull"):
+    """A rigid body whose collision geom is the convex hull of ``points`` (given about the body origin), with the
+    link frame at the hull's centre of mass (``rigid_entity.py:925-1007`` convexify, ``:958-1047`` ``_align_link``)."""
+    from .specs import GEOM_MESH
+
+    verts, faces = mu.convex_hull(np.asarray(points, dtype=np.float64))
+    vol, com, inertia = mu.mass_properties(verts, faces)
+    verts = verts - com
+    pos = np.asarray(pos, dtype=np.float64) + tf.quat_to_R(tf.quat_normalize(quat)) @ com
+    geom = GeomSpec(
+        type=GEOM_MESH, pos=np.zeros(3), quat=np.array([1.0, 0, 0, 0]), data=np.zeros(7), verts=verts, faces=faces,
+        friction=friction, sol_params=default_sol_params(), center=np.zeros(3), name=name,
+    )
+    return _free_body(name, pos, quat, rho * vol, rho * inertia, geom, fixed)

@@ -34,7 +34,7 @@ ERR_INVALID_ACC_NAN = 1 << 4
 COUNT_KINDS = [
     "ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "EQUALITIES", "DOFS2", "GEOMS2", "PAIRS",
     "SUPPORT", "CONTACTS", "CONSTRAINTS", "CONSTRAINTS_DOFS", "BROAD", "SORT", "WORK", "SCHED", "DOFS_PLANE",
-    "ONE_PLANE",
+    "ONE_PLANE", "VERTS",
 ]
 ENV_MAJOR_KINDS = {"WORK"}
 LOCAL_KINDS = {"ONE", "LINKS", "JOINTS", "DOFS", "QS", "GEOMS", "ENTITIES", "DOFS2", "SORT"}
@@ -115,7 +115,7 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.rs_abi_version() != 1:
+    if lib.rs_abi_version() != 2:
         raise RigidStepError("librigidstep ABI version mismatch")
     _lib = lib
     return lib

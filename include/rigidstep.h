@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define RS_ABI_VERSION 1
+#define RS_ABI_VERSION 2
 
 enum {
     RS_OK = 0,
@@ -78,6 +78,7 @@ enum {
     RS_N_SCHED, /* fused-substep schedule: 2 x 4 cost-class queues of environment ids and one control row */
     RS_N_DOFS_PLANE, /* n_dofs, always a [row][env] plane (state the fused substep keeps outside its private region) */
     RS_N_ONE_PLANE,  /* 1, likewise */
+    RS_N_VERTS,      /* collision-geometry vertices (verts_info.init_pos of the reference) */
     RS_N_KINDS
 };
 
@@ -96,7 +97,7 @@ typedef struct rs_model_header {
     int32_t abi_version;     /* RS_ABI_VERSION */
     int32_t float_bytes;     /* 4 for the device blob */
     int32_t n_links, n_joints, n_dofs, n_qs, n_geoms, n_entities, n_equalities;
-    int32_t n_pairs, n_support;
+    int32_t n_pairs, n_support, n_verts;
     int32_t max_collision_pairs, max_contact_pairs, max_collision_pairs_broad, n_contacts_per_pair;
     int32_t len_constraints;
     /* options (RigidOptions / SimOptions subset, genesis/options/solvers.py:482-535) */

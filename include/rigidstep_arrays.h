@@ -5,7 +5,8 @@
  * offset given by rs_model_header.array_offset[index]; element (i, c) of an array is at [i * width + c].
  * Floats are rs_model_header.float_bytes wide. Contents follow the reference solver's info structs
  * (genesis/utils/array_class.py: LinksInfo, JointsInfo, DofsInfo, GeomsInfo, EntitiesInfo, EqualitiesInfo,
- * RigidGlobalInfo.mass_parent_mask, ColliderInfo.collision_pair_idx / valid_collision_pairs, SupportFieldInfo).
+ * RigidGlobalInfo.mass_parent_mask, ColliderInfo.collision_pair_idx / valid_collision_pairs, SupportFieldInfo,
+ * VertsInfo.init_pos with GeomsInfo.vert_start / vert_end).
  */
 #ifndef RIGIDSTEP_ARRAYS_H
 #define RIGIDSTEP_ARRAYS_H
@@ -63,6 +64,8 @@
     RS_MA(geom_is_convex, 1, RS_N_GEOMS, 1)                      \
     RS_MA(geom_is_fixed, 1, RS_N_GEOMS, 1)                       \
     RS_MA(geom_support_start, 1, RS_N_GEOMS, 1)                  \
+    RS_MA(geom_vert_start, 1, RS_N_GEOMS, 1)                     \
+    RS_MA(geom_vert_end, 1, RS_N_GEOMS, 1)                       \
     RS_MA(geom_pos, 0, RS_N_GEOMS, 3)                            \
     RS_MA(geom_quat, 0, RS_N_GEOMS, 4)                           \
     RS_MA(geom_data, 0, RS_N_GEOMS, 7)                           \
@@ -78,6 +81,8 @@
     RS_MA(mass_parent_mask, 0, RS_N_DOFS2, 1)                    \
     RS_MA(collision_pair_idx, 1, RS_N_GEOMS2, 1)                 \
     RS_MA(valid_pairs, 1, RS_N_PAIRS, 2)                         \
-    RS_MA(support_v, 0, RS_N_SUPPORT, 3)
+    RS_MA(support_v, 0, RS_N_SUPPORT, 3)                         \
+    RS_MA(support_vid, 1, RS_N_SUPPORT, 1)                       \
+    RS_MA(verts, 0, RS_N_VERTS, 3)
 
 #endif /* RIGIDSTEP_ARRAYS_H */

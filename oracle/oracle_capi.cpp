@@ -172,6 +172,27 @@ template <class T> static int with_ctx(void* h, std::function<int(Context<T>&)> 
          ? with_ctx<double>(h, [&](Context<double>& c) -> int { body })                \
          : with_ctx<float>(h, [&](Context<float>& c) -> int { body }))
 
+template <class T>
+static int pair_contact(Context<T>& c, int ga, int gb, const double* pose_a, const double* pose_b, int mpr,
+                        double* out) {
+    V3<T> pa = v3(T(pose_a[0]), T(pose_a[1]), T(pose_a[2]));
+    Q4<T> qa{T(pose_a[3]), T(pose_a[4]), T(pose_a[5]), T(pose_a[6])};
+    V3<T> pb = v3(T(pose_b[0]), T(pose_b[1]), T(pose_b[2]));
+    Q4<T> qb{T(pose_b[3]), T(pose_b[4]), T(pose_b[5]), T(pose_b[6])};
+    V3<T> n = v3(T(0), T(0), T(0)), p = n;
+    T pen = T(0);
+    bool col = false;
+    if (mpr) {
+        mpr_contact(c.m, ga, gb, n, pa, qa, pb, qb, col, n, pen, p);
+    } else {
+        static thread_local GJKState<T> gjk;
+        col = gjk_contact(c.m, gjk, ga, gb, pa, qa, pb, qb, n, p, pen);
+    }
+    for (int k = 0; k < 3; k++) out[k] = double(n[k]), out[3 + k] = double(p[k]);
+    out[6] = double(pen);
+    return col ? 1 : 0;
+}
+
 extern "C" {
 
 void* orc_create(const rs_model_header* hdr, const void* blob, int precision, int n_envs) {
@@ -289,6 +310,17 @@ int orc_step(void* h, int n_steps, int n_threads) {
         }
         return 0;
     });
+}
+
+// One detection of geoms ga, gb of the model at the given world poses (pose = x, y, z, qw, qx, qy, qz) with the
+// GJK/EPA path (gjk.py:170-446). out = normal[3], position[3], penetration; returns 1 when they collide. With
+// mpr = 1 the MPR path (mpr.py:769-818) runs instead, without a warm-start direction.
+int orc_pair_contact(void* h, int ga, int gb, const double* pose_a, const double* pose_b, int mpr, double* out) {
+    if (reinterpret_cast<Handle*>(h)->precision == 64)
+        return pair_contact<double>(*reinterpret_cast<Context<double>*>(reinterpret_cast<Handle*>(h)->ctx), ga, gb,
+                                    pose_a, pose_b, mpr, out);
+    return pair_contact<float>(*reinterpret_cast<Context<float>*>(reinterpret_cast<Handle*>(h)->ctx), ga, gb, pose_a,
+                               pose_b, mpr, out);
 }
 
 }  // extern "C"

@@ -1,9 +1,9 @@
 // ORACLE (test infrastructure only): collision detection of one environment.
 // Restates genesis/engine/solvers/rigid/collider/{broadphase,narrowphase,mpr,support_field,contact,box_contact,
-// capsule_contact,utils}.py on the CPU code path (SAP broadphase + monolithic narrowphase, collider.py:684-808).
-// The GJK/EPA fallback of the MPR path (narrowphase.py:726-734) is not restated: when it would trigger, the MPR result
-// is kept (this restatement and the device kernels share that limitation).
+// capsule_contact,utils}.py on the CPU code path (SAP broadphase + monolithic narrowphase, collider.py:684-808),
+// including the GJK/EPA hand-over of the MPR path (narrowphase.py:726-851, oracle_gjk.h).
 #pragma once
+#include "oracle_gjk.h"
 #include "oracle_model.h"
 
 namespace orc {
@@ -550,7 +550,7 @@ bool sphere_capsule(const Model<T>& m, int ga, int gb, V3<T> pa_, Q4<T> qa, V3<T
     return col;
 }
 
-// narrowphase.py:520-970 func_convex_convex_contact (MPR code path)
+// narrowphase.py:520-970 func_convex_convex_contact (MPR with the GJK/EPA hand-over)
 template <class T> void convex_convex_contact(const Model<T>& m, Env<T>& s, int ga, int gb) {
     using E = Env<T>;
     int ta = m.geom_type[ga], tb = m.geom_type[gb];
@@ -604,12 +604,22 @@ template <class T> void convex_convex_contact(const Model<T>& m, Env<T>& s, int 
                         updated = true;
                     }
                 }
-                // narrowphase.py:729-734: a deep first penetration (no cached direction) or a large overlap would go
-                // to GJK/EPA in the reference; only counted here
+                // narrowphase.py:726-734: a deep penetration with no cached direction, or a large overlap, is
+                // detected again with GJK/EPA (narrowphase.py:766-851; a GJK detection gives one contact, whose
+                // normal, position and depth replace MPR's when the geoms collide)
                 if (pen > tolerance &&
                     (!avail || T(m.h.mc_tolerance) * pen >= T(m.h.mpr_to_gjk_overlap_ratio) * tolerance)) {
                     if (det == 0) s.n_gjk_regime++;
                     else s.n_gjk_regime_perturbed++;
+                    static thread_local GJKState<T> gjk;
+                    V3<T> gn, gp;
+                    T gpen;
+                    is_col = gjk_contact(m, gjk, ga, gb, pa, qa, pb, qb, gn, gp, gpen);
+                    pen = gpen;
+                    if (is_col) {
+                        n = gn;
+                        cp = gp;
+                    }
                 }
             }
         }

@@ -55,6 +55,7 @@ inline int64_t count_of(const rs_model_header& h, int kind) {
         case RS_N_GEOMS2: return (int64_t)h.n_geoms * h.n_geoms;
         case RS_N_PAIRS: return h.n_pairs;
         case RS_N_SUPPORT: return h.n_support;
+        case RS_N_VERTS: return h.n_verts;
         default: return 0;
     }
 }

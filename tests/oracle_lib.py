@@ -38,6 +38,9 @@ def load():
         lib.orc_set_options.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_set_options.restype = ctypes.c_int
         lib.orc_n_envs.argtypes = [ctypes.c_void_p]
+        lib.orc_pair_contact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_pair_contact.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -91,6 +94,16 @@ class Oracle:
 
     def stage(self, stage: int):
         _lib.orc_stage(self.h, stage)
+
+    def pair_contact(self, ga: int, gb: int, pose_a, pose_b, method: str = "gjk"):
+        """One detection of geoms ga, gb at world poses (x, y, z, qw, qx, qy, qz) with GJK/EPA or MPR:
+        (collides, normal, position, penetration)."""
+        pa = np.ascontiguousarray(pose_a, np.float64)
+        pb = np.ascontiguousarray(pose_b, np.float64)
+        out = np.zeros(7, np.float64)
+        col = _lib.orc_pair_contact(self.h, ga, gb, pa.ctypes.data, pb.ctypes.data, int(method == "mpr"),
+                                    out.ctypes.data)
+        return bool(col), out[:3].copy(), out[3:6].copy(), float(out[6])
 
     def step(self, n_steps: int = 1, n_threads: int = 1):
         _lib.orc_step(self.h, n_steps, n_threads)
