@@ -2,10 +2,11 @@
 // Restates genesis/engine/solvers/rigid/collider/{broadphase,narrowphase,mpr,support_field,contact,box_contact,
 // capsule_contact,utils}.py: warm-started sweep-and-prune, then the monolithic convex narrowphase (MPR with cached
 // normal, 4 perturbed re-detections for multi-contact), then the plane-box specialisation. Contacts are appended in the
-// broadphase pair order, which makes the contact list deterministic.
-// Not built: the GJK/EPA fallback of the MPR path (narrowphase.py:726-734); the MPR result is kept instead.
+// broadphase pair order, which makes the contact list deterministic. Deep or poorly warm-started MPR detections are
+// handed to GJK/EPA (narrowphase.py:726-851, rs_gjk.h).
 #pragma once
 #include "rs_device.h"
+#include "rs_gjk.h"
 
 namespace rs {
 
@@ -640,11 +641,15 @@ __device__ inline void convex_pose(const DModel& m, const S& st, int b, int ga, 
     }
 }
 
-// one detection at poses (pa, qa), (pb, qb); detection 0 retries MPR without the warm-start normal
+// one detection at poses (pa, qa), (pb, qb); detection 0 retries MPR without the warm-start normal. want_gjk is set
+// when the MPR answer is to be replaced by a GJK/EPA detection at the same poses (narrowphase.py:726-734); the caller
+// runs it (one lane alone, or the whole wavefront) and applies it with apply_gjk.
 __device__ inline bool convex_detect(const DModel& m, const DState& gs, int ge, int ga, int gb, int pair, bool first,
-                                     f3 pa, f4q qa, f3 pb, f4q qb, f3& n, f3& cp, float& pen) {
+                                     float tolerance, f3 pa, f4q qa, f3 pb, f4q qb, f3& n, f3& cp, float& pen,
+                                     bool& want_gjk) {
     const int ta = m.geom_type[ga], tb = m.geom_type[gb];
     bool col = false;
+    want_gjk = false;
     if (ta == GT_CAPSULE && tb == GT_CAPSULE) {
         col = capsule_capsule(m, ga, gb, pa, qa, pb, qb, n, cp, pen);
     } else if ((ta == GT_SPHERE && tb == GT_CAPSULE) || (ta == GT_CAPSULE && tb == GT_SPHERE)) {
@@ -658,17 +663,30 @@ __device__ inline bool convex_detect(const DModel& m, const DState& gs, int ge, 
         col = pen > 0.f;
     } else {
         f3 nws = ld3(gs.cache_normal, pair, gs.B, ge);
-        const bool avail = fabsf(nws.x) > m.eps || fabsf(nws.y) > m.eps || fabsf(nws.z) > m.eps;
+        bool avail = fabsf(nws.x) > m.eps || fabsf(nws.y) > m.eps || fabsf(nws.z) > m.eps;
 #pragma clang loop unroll(disable)
         for (int i_mpr = 0; i_mpr < 2; i_mpr++) {
             if (i_mpr == 1) {
                 if (!(first && !col && avail)) break;
                 nws = zero3();
+                avail = false;
             }
             mpr_contact(m, ga, gb, nws, pa, qa, pb, qb, col, n, pen, cp);
         }
+        want_gjk = prefer_gjk(m, pen, tolerance, avail);
     }
     return col;
+}
+
+// narrowphase.py:799-851: the GJK/EPA detection replaces MPR's penetration, and its normal and position when it
+// found a contact
+__device__ __forceinline__ bool apply_gjk(bool gcol, f3 gn, f3 gp, float gpen, f3& n, f3& cp, float& pen) {
+    pen = gpen;
+    if (gcol) {
+        n = gn;
+        cp = gp;
+    }
+    return gcol;
 }
 
 template <class Sink, class S>
@@ -714,7 +732,7 @@ __device__ inline void convex_merge(Sink& sink, int& n_con, int ga, int gb, cons
 
 template <class Sink, class S>
 __device__ inline void convex_convex(const DModel& m, const S& st, int b, const DState& gs, int ge, int ga, int gb,
-                                     Sink& sink) {
+                                     Sink& sink, float* gjk_ws) {
     ConvexFirst f;
     if (!convex_setup(m, ga, gb, f)) return;
     int n_con = 1;
@@ -726,7 +744,16 @@ __device__ inline void convex_convex(const DModel& m, const S& st, int b, const 
         convex_pose(m, st, b, ga, gb, f, det, pa, qa, pb, qb, qr);
         f3 n = zero3(), cp = zero3();
         float pen = 0.f;
-        const bool col = convex_detect(m, gs, ge, ga, gb, f.pair, det == 0, pa, qa, pb, qb, n, cp, pen);
+        bool want_gjk;
+        bool col = convex_detect(m, gs, ge, ga, gb, f.pair, det == 0, f.tolerance, pa, qa, pb, qb, n, cp, pen,
+                                 want_gjk);
+        if (want_gjk) {
+            const GjkPair gp = make_gjk_pair(m, ga, gb, pa, qa, pb, qb);
+            f3 gn = zero3(), gpos = zero3();
+            float gpen = 0.f;
+            const bool gcol = gjk_contact_mem(m, gp, gjk_ws, gn, gpos, gpen);
+            col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
+        }
         if (det == 0) {
             convex_first(m, st, b, gs, ge, ga, gb, col, n, cp, pen, sink, f);
         } else if (col) {
@@ -776,6 +803,7 @@ __device__ inline bool convex_pair(const DModel& m, int ta, int tb, int ga, int 
 template <class S>
 __device__ inline void dev_narrowphase(const DModel& m, const S& st, int b) {
     PlaneSink sink{m, st, b};
+    float* gjk_ws = st.wk + (size_t)b * m.work_stride + m.off_gjk;
     int nb = SF(n_broad, 0);
     for (int i = 0; i < nb; i++) {
         int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);
@@ -785,7 +813,7 @@ __device__ inline void dev_narrowphase(const DModel& m, const S& st, int b) {
             gb = t;
         }
         int ta = m.geom_type[ga], tb = m.geom_type[gb];
-        if (convex_pair(m, ta, tb, ga, gb)) convex_convex(m, st, b, st, b, ga, gb, sink);
+        if (convex_pair(m, ta, tb, ga, gb)) convex_convex(m, st, b, st, b, ga, gb, sink, gjk_ws);
     }
     for (int i = 0; i < nb; i++) {
         int ga = SF(broad_pairs, 2 * i), gb = SF(broad_pairs, 2 * i + 1);

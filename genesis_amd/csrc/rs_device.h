@@ -23,6 +23,9 @@ struct DModel {
     float dt, gx, gy, gz, tolerance, ls_tolerance, meaninertia, mc_perturbation, mc_tolerance, eps;
     int iterations, ls_iterations, enable_collision, enable_joint_limit, disable_constraint, enable_multi_contact;
     int box_box_detection, integrator, solver_type;
+    float mpr_to_gjk_overlap_ratio;
+    int64_t work_stride;  // floats of the work field per environment
+    int off_gjk;          // offset of the per-stage kernels' GJK/EPA polytope in an environment's work slice
 #define RS_DM_PTR(name, is_int, kind, width) const typename std::conditional<is_int, int32_t, float>::type RS_CONST* name;
     RS_MODEL_ARRAYS(RS_DM_PTR)
 #undef RS_DM_PTR

@@ -166,9 +166,11 @@ __device__ inline void wave_broadphase(const DModel& m, const LState& loc, const
 // convex pairs first and plane-box pairs second - the order in which the sequential narrowphase appends them. The
 // four perturbed detections of each multi-contact pair that collided run one per lane (each lane takes a (pair,
 // detection) item); the pair's lane then merges their results in detection order, as the sequential loop does.
-// mc: per-pair records (kMcRec floats per lane), the item map and the item results, in the work field.
-__device__ inline void wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
-                                        float* mc, int lane) {
+// mc: per-pair records (kMcRec floats per lane), the item map and the item results, in the work field. Detections
+// handed to GJK/EPA run on the whole wavefront when there is one (polytope in gjk_lds, LDS or nullptr; in gjk_mem when
+// it outgrows that) and one per lane when there are several (polytopes in the kGjkLaneSlots slots at gjk_mem).
+__device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
+                                        float* mc, float* gjk_lds, float* gjk_mem, int lane) {
     const int nb = FP(loc, n_broad)[0];
     int placed = 0;  // uniform: contacts produced so far
     SlotSink sink{pc + (size_t)lane * m.NCPP * 8, m.NCPP, 0};
@@ -221,16 +223,49 @@ __device__ inline void wave_narrowphase(const DModel& m, const LState& loc, cons
                             g.pair = ((const int*)r)[12];
                             dga = ((const int*)r)[13];
                             dgb = ((const int*)r)[14];
+                            g.tolerance = r[15];
                         }
                         f3 n = zero3(), cp = zero3();
                         float pen = 0.f;
                         f4q qr{0.f, 0.f, 0.f, 0.f};
-                        bool col = false;
+                        bool col = false, want_gjk = false;
+                        f3 pa = zero3(), pb = zero3();
+                        f4q qa = qident(), qb = qident();
                         if (act) {
-                            f3 pa, pb;
-                            f4q qa, qb;
                             convex_pose(m, loc, 0, dga, dgb, g, det, pa, qa, pb, qb, qr);
-                            col = convex_detect(m, st, b, dga, dgb, g.pair, det == 0, pa, qa, pb, qb, n, cp, pen);
+                            col = convex_detect(m, st, b, dga, dgb, g.pair, det == 0, g.tolerance, pa, qa, pb, qb, n,
+                                                cp, pen, want_gjk);
+                        }
+                        // the detections handed to GJK/EPA: a single one runs on the whole wavefront (polytope in
+                        // LDS); several run one per lane, kGjkLaneSlots at a time, each with its polytope in a lane
+                        // slot of the work field; the lane that owns a detection takes its result
+                        uint64_t gm = wave_ballot(act && want_gjk);
+                        if (gm && !(gm & (gm - 1))) {
+                            const int l = __builtin_ctzll(gm);
+                            const GjkPair gp = make_gjk_pair(m, rl_i(dga, l), rl_i(dgb, l), rl3(pa, l), rl4(qa, l),
+                                                             rl3(pb, l), rl4(qb, l));
+                            f3 gn = zero3(), gpos = zero3();
+                            float gpen = 0.f;
+                            const bool gcol = gjk_contact_wave(m, gp, lane, gjk_lds, gjk_mem, gn, gpos, gpen);
+                            if (lane == l) col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
+                        } else {
+#pragma clang loop unroll(disable)
+                            while (gm) {
+                                uint64_t pass = 0;
+                                for (int k = 0; k < kGjkLaneSlots && gm; k++) {
+                                    pass |= gm & (~gm + 1);
+                                    gm &= gm - 1;
+                                }
+                                if ((pass >> lane) & 1) {
+                                    const int slot = __popcll(pass & ((1ull << lane) - 1));
+                                    const GjkPair gp = make_gjk_pair(m, dga, dgb, pa, qa, pb, qb);
+                                    f3 gn = zero3(), gpos = zero3();
+                                    float gpen = 0.f;
+                                    const bool gcol = gjk_contact_mem(m, gp, gjk_mem + (size_t)slot * kGjkMemWords, gn,
+                                                                      gpos, gpen);
+                                    col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
+                                }
+                            }
                         }
                         if (round == 0) {
                             if (act) convex_first(m, loc, 0, st, b, ga, gb, col, n, cp, pen, sink, f);
@@ -255,6 +290,7 @@ __device__ inline void wave_narrowphase(const DModel& m, const LState& loc, cons
                             ((int*)r)[12] = f.pair;
                             ((int*)r)[13] = ga;
                             ((int*)r)[14] = gb;
+                            r[15] = f.tolerance;
                             for (int j = 0; j < 4; j++) imap[first_item + j] = lane * 4 + j;
                         }
                     }
@@ -322,7 +358,8 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
             const int n_old = FP(loc, n_contacts)[0];
             for (int i = lane; i < n_old; i += kWave) clear_contacts(st, b, i, i + 1);
             wave_broadphase(m, loc, st, b, lane);
-            wave_narrowphase(m, loc, st, b, wk + fa.off_pc, wk + fa.off_mc, lane);
+            wave_narrowphase(m, loc, st, b, wk + fa.off_pc, wk + fa.off_mc, fa.gjk_off >= 0 ? slice + fa.gjk_off : nullptr,
+                             wk + fa.off_gjk, lane);
         }
     }
     // ConstraintSolver.resolve

@@ -137,6 +137,8 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->loc_floats = (int)off;
     fa->tiled = tile_class(h.n_dofs);
     fa->tile_off = 0;  // set per launch by set_lds_slice
+    fa->gjk_words = 0;  // set at model creation (models with mesh geoms)
+    fa->gjk_off = -1;
     fa->lds_wave = (int)up4((int64_t)kTileStride * tile_rows(h.n_dofs));
     fa->sched = 0;
     const int64_t C = h.len_constraints > 0 ? h.len_constraints : 1, D = h.n_dofs;
@@ -146,19 +148,22 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->off_tmp = (int)(fa->off_rowf + RW_COUNT * up4(C));
     fa->off_pc = (int)(fa->off_tmp + kTmpArrays * up4(C));
     fa->off_mc = (int)(fa->off_pc + up4((int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8));
-    int64_t total = fa->off_mc + kMcFloats;
+    // GJK/EPA polytopes: kGjkLaneSlots slots (the per-stage kernels use the first)
+    fa->off_gjk = (int)up4(fa->off_mc + kMcFloats);
+    int64_t total = fa->off_gjk + (int64_t)kGjkLaneSlots * kGjkMemWords;
     fa->work_stride = (total + 63) / 64 * 64;
 }
 
 // LDS slice of one wavefront: the private region (when it is in LDS) followed by the row tile
 static int tile_floats(const rs_model_header& h) { return (kTileStride * tile_rows(h.n_dofs) + 3) / 4 * 4; }
+// slice layout: [private region (LDS mode)][row tile][GJK/EPA polytope (models with mesh geoms)]
 static void set_lds_slice(const rs_model_header& h, FusedArgs* fa, bool use_lds, int slice_floats) {
     fa->tile_off = use_lds ? fa->loc_floats : 0;
+    fa->gjk_off = fa->gjk_words > 0 ? fa->tile_off + tile_floats(h) : -1;
     fa->lds_wave = slice_floats;
-    (void)h;
 }
 static int lds_slice_floats(const rs_model_header& h, const FusedArgs& fa, bool use_lds) {
-    return (use_lds ? fa.loc_floats : 0) + tile_floats(h);
+    return (use_lds ? fa.loc_floats : 0) + tile_floats(h) + fa.gjk_words;
 }
 
 static int64_t work_floats(const rs_model_header& h) {
@@ -205,9 +210,9 @@ static void layout(const rs_model* m, int n_envs, int64_t* offs, int64_t* total)
 }
 
 static DState make_state(const rs_model* m, void* base, int n_envs) {
-    std::vector<int64_t> offs(k_n_state_fields);
+    int64_t offs[k_n_state_fields];
     int64_t total;
-    layout(m, n_envs, offs.data(), &total);
+    layout(m, n_envs, offs, &total);
     DState s;
     s.B = n_envs;
     s.LS = record_floats(m);
@@ -296,6 +301,68 @@ __global__ void k_errno_or(DState st, int32_t* out) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     int v = b < st.B ? FV(st, errbits, b)[0] : 0;
     if (v) atomicOr(out, v);
+}
+
+// ------------------------------------------------------------------------------- single detections (rs_pair_contact)
+__device__ __forceinline__ bool pair_item(const DModel& m, int n, int i, const int32_t* ga, const int32_t* gb,
+                                          const float* pa, const float* pb, GjkPair& p) {
+    if (i >= n) return false;
+    const int g1 = ga[i], g2 = gb[i];
+    if (g1 < 0 || g1 >= m.G || g2 < 0 || g2 >= m.G) return false;
+    const int ta = m.geom_type[g1], tb = m.geom_type[g2];
+    if (ta == GT_PLANE || tb == GT_PLANE || ta == GT_TERRAIN || tb == GT_TERRAIN) return false;
+    const float* a = pa + 7 * (size_t)i;
+    const float* c = pb + 7 * (size_t)i;
+    p = make_gjk_pair(m, g1, g2, mk3(a[0], a[1], a[2]), f4q{a[3], a[4], a[5], a[6]}, mk3(c[0], c[1], c[2]),
+                      f4q{c[3], c[4], c[5], c[6]});
+    return true;
+}
+
+__device__ __forceinline__ void pair_out(int i, bool valid, bool col, f3 nrm, f3 pos, float pen, float* out,
+                                         int32_t* hit) {
+    float* o = out + 7 * (size_t)i;
+    o[0] = nrm.x; o[1] = nrm.y; o[2] = nrm.z;
+    o[3] = pos.x; o[4] = pos.y; o[5] = pos.z;
+    o[6] = pen;
+    hit[i] = valid ? (col ? 1 : 0) : -1;
+}
+
+// one wavefront per item: the fused kernel's GJK/EPA (polytope in the lanes' registers)
+__global__ void __launch_bounds__(kWave) k_pair_gjk_wave(DModel m, int n, const int32_t* ga, const int32_t* gb,
+                                                         const float* pa, const float* pb, float* work, float* out,
+                                                         int32_t* hit) {
+    __shared__ __align__(16) float lds[kGjkLdsWords];
+    const int i = blockIdx.x;
+    GjkPair p;
+    const bool valid = pair_item(m, n, i, ga, gb, pa, pb, p);
+    if (i >= n) return;
+    f3 nrm = zero3(), pos = zero3();
+    float pen = 0.f;
+    const bool col =
+        valid ? gjk_contact_wave(m, p, lane_id(), lds, work + (size_t)i * kGjkMemWords, nrm, pos, pen) : false;
+    if (lane_id() == 0) pair_out(i, valid, col, nrm, pos, pen, out, hit);
+}
+
+// one lane per item: the per-stage kernels' GJK/EPA (polytope in work, kGjkMemWords floats per item) or MPR without a
+// warm-start direction
+__global__ void __launch_bounds__(kBlock) k_pair_lane(DModel m, int n, const int32_t* ga, const int32_t* gb,
+                                                      const float* pa, const float* pb, int method, float* work,
+                                                      float* out, int32_t* hit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    GjkPair p;
+    const bool valid = pair_item(m, n, i, ga, gb, pa, pb, p);
+    if (i >= n) return;
+    f3 nrm = zero3(), pos = zero3();
+    float pen = 0.f;
+    bool col = false;
+    if (valid) {
+        if (method == RS_PAIR_GJK) {
+            col = gjk_contact_mem(m, p, work + (size_t)i * kGjkMemWords, nrm, pos, pen);
+        } else {
+            mpr_contact(m, p.ga, p.gb, zero3(), p.pa, p.qa, p.pb, p.qb, col, nrm, pen, pos);
+        }
+    }
+    pair_out(i, valid, col, nrm, pos, pen, out, hit);
 }
 
 // ------------------------------------------------------------------------------------------------ launch helpers
@@ -395,6 +462,19 @@ int rs_state_field_info(int i, rs_array_info* out) {
     return RS_OK;
 }
 
+// whether the host blob holds a convex mesh geom (geom_type == GT_MESH and geom_is_convex)
+static bool has_convex_mesh(const rs_model_header& h, const char* blob) {
+    const int32_t *type = nullptr, *convex = nullptr;
+    for (int i = 0; i < k_n_model_arrays; i++) {
+        if (!strcmp(k_model_arrays[i].name, "geom_type")) type = (const int32_t*)(blob + h.array_offset[i]);
+        if (!strcmp(k_model_arrays[i].name, "geom_is_convex")) convex = (const int32_t*)(blob + h.array_offset[i]);
+    }
+    if (!type || !convex) return false;
+    for (int g = 0; g < h.n_geoms; g++)
+        if (type[g] == GT_MESH && convex[g]) return true;
+    return false;
+}
+
 int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_bytes, rs_model** out) {
     if (!h || !blob || !out) return fail(RS_E_INVALID, "rs_model_create: null argument");
     if (h->abi_version != RS_ABI_VERSION) return fail(RS_E_VERSION, "rs_model_create: ABI version mismatch");
@@ -428,6 +508,7 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
     d.dt = h->dt; d.gx = h->gravity[0]; d.gy = h->gravity[1]; d.gz = h->gravity[2];
     d.tolerance = h->tolerance; d.ls_tolerance = h->ls_tolerance; d.meaninertia = h->meaninertia;
     d.mc_perturbation = h->mc_perturbation; d.mc_tolerance = h->mc_tolerance;
+    d.mpr_to_gjk_overlap_ratio = h->mpr_to_gjk_overlap_ratio;
     d.eps = 1.1920928955078125e-07f;
     d.iterations = h->iterations; d.ls_iterations = h->ls_iterations;
     d.enable_collision = h->enable_collision; d.enable_joint_limit = h->enable_joint_limit;
@@ -449,6 +530,11 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
     }
     static_assert(k_n_state_fields <= kMaxStateFields, "FusedArgs::loc_off too small");
     fused_layout(m->h, &m->fa);
+    d.work_stride = m->fa.work_stride;
+    d.off_gjk = m->fa.off_gjk;
+    // models with convex mesh geoms meet the GJK/EPA hand-over often (deep overlaps of hulls): their wavefronts keep
+    // a small polytope in LDS; the others run the rare detection with the polytope in the work field
+    m->fa.gjk_words = has_convex_mesh(*h, (const char*)blob) ? kGjkLdsWords : 0;
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device) != hipSuccess) max_lds = 0;
     m->max_lds = max_lds < 65536 ? max_lds : 65536;
@@ -593,6 +679,29 @@ int rs_errno_reduce(const rs_model* m, const void* state, int n_envs, int32_t* o
     HIP_OK(hipMemcpyAsync(out, dv, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_OK(hipFreeAsync(dv, s));
     HIP_OK(hipStreamSynchronize(s));
+    return RS_OK;
+}
+
+int64_t rs_pair_contact_work_floats(void) { return kGjkMemWords; }
+
+int rs_pair_contact(const rs_model* m, int n, const int32_t* ga, const int32_t* gb, const float* pose_a,
+                    const float* pose_b, int method, int lanes, float* work, float* out, int32_t* hit, void* stream) {
+    if (!m || n < 0 || (n > 0 && (!ga || !gb || !pose_a || !pose_b || !out || !hit)))
+        return fail(RS_E_INVALID, "rs_pair_contact: bad argument");
+    if (method != RS_PAIR_GJK && method != RS_PAIR_MPR) return fail(RS_E_INVALID, "rs_pair_contact: bad method");
+    if (lanes != 1 && lanes != kWave) return fail(RS_E_INVALID, "rs_pair_contact: lanes must be 1 or 64");
+    if (lanes == kWave && method != RS_PAIR_GJK)
+        return fail(RS_E_INVALID, "rs_pair_contact: the wavefront detection is GJK/EPA only");
+    if (method == RS_PAIR_GJK && n > 0 && !work) return fail(RS_E_INVALID, "rs_pair_contact: GJK/EPA needs a work buffer");
+    if (n == 0) return RS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (lanes == kWave)
+        hipLaunchKernelGGL(k_pair_gjk_wave, dim3((unsigned)n), dim3(kWave), 0, s, m->dm, n, ga, gb, pose_a, pose_b,
+                           work, out, hit);
+    else
+        hipLaunchKernelGGL(k_pair_lane, env_grid(n), dim3(kBlock), 0, s, m->dm, n, ga, gb, pose_a, pose_b, method,
+                           work, out, hit);
+    HIP_OK(hipGetLastError());
     return RS_OK;
 }
 

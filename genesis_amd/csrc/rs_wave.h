@@ -25,7 +25,9 @@ struct FusedArgs {
     int tiled;         // row-tile solver class: kTileMaxDofs or kTileWideDofs (the KD its variants are built for), 0: none
     int lds_wave;      // floats of LDS per wavefront: [private region if in LDS][row tile]
     int64_t work_stride;  // floats of the work field per environment
-    int off_loc, off_jac, off_rowf, off_tmp, off_pc, off_mc;
+    int off_loc, off_jac, off_rowf, off_tmp, off_pc, off_mc, off_gjk;
+    int gjk_words;     // LDS floats per wavefront for the GJK/EPA polytope (0: the polytope stays in the work field)
+    int gjk_off;       // offset of that polytope in the wavefront's LDS slice (-1: none)
     int tile_off;      // offset of the row tile (kTileStride x tile_rows(D) floats) in the wavefront's LDS slice
     int sched;         // 1: map wavefronts to environments through the cost-ordered queues
     int loc_off[kMaxStateFields];  // private-region offset of each state field (local kinds only)

@@ -1092,6 +1092,9 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
         if (isnan(qa)) LF(errbits, 0) = LF(errbits, 0) | RS_ERR_INVALID_FORCE_NAN;
     }
     LF(is_warmstart, 0) = 1;
+    // the constraint forces of the rows (ConstraintState.efc_force), from the work field to the state slab
+    const int n_rows = LF(n_con, 0);
+    for (int c = lane; c < n_rows; c += kWave) st.efc_force[(size_t)c * st.B + b] = R.force(c);
     const int cs = LF(n_con_eq, 0) + LF(n_con_fl, 0);
     const int ncont = LF(n_contacts, 0);
     for (int i = lane; i < ncont; i += kWave) {

@@ -21,6 +21,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librigidstep.so")
 
 RS_MODE_PLAIN, RS_MODE_FUSED = 0, 1
+RS_OK, RS_E_INVALID, RS_E_DEVICE, RS_E_VERSION, RS_E_UNSUPPORTED = 0, -1, -2, -3, -4
+RS_PAIR_GJK, RS_PAIR_MPR = 0, 1
 STAGES = ["fk", "crba", "factor", "rne", "aabb", "broadphase", "narrowphase", "constraints", "solve", "integrate"]
 STAGE_ID = {n: i for i, n in enumerate(STAGES)}
 
@@ -91,6 +93,10 @@ _SIGNATURES = {
                                        ctypes.c_void_p]),
     "rs_errno_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]),
+    "rs_pair_contact_work_floats": (ctypes.c_int64, []),
+    "rs_pair_contact": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "rs_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "rs_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -124,6 +130,28 @@ def load_library(path: str | None = None):
 def _check(code: int):
     if code != 0:
         raise RigidStepError(f"librigidstep error {code}: {_lib.rs_last_error().decode()}")
+
+
+def pair_contact(dmodel, ga, gb, pose_a, pose_b, method: int = RS_PAIR_GJK, lanes: int = 64, stream=None):
+    """One detection per item on the device (rs_pair_contact; tests and diagnostics). ga, gb: int sequences of geom
+    indices; pose_a, pose_b: [n, 7] world poses (x, y, z, qw, qx, qy, qz). Returns (hit [n] int32, out [n, 7] float32)
+    as device tensors after the launch (hit: 1 contact, 0 none, -1 invalid item)."""
+    load_library()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    ga_t = torch.as_tensor(ga, dtype=torch.int32).to(dev).contiguous()
+    gb_t = torch.as_tensor(gb, dtype=torch.int32).to(dev).contiguous()
+    pa = torch.as_tensor(pose_a, dtype=torch.float32).to(dev).contiguous()
+    pb = torch.as_tensor(pose_b, dtype=torch.float32).to(dev).contiguous()
+    n = int(ga_t.numel())
+    out = torch.zeros((n, 7), dtype=torch.float32, device=dev)
+    hit = torch.full((n,), -2, dtype=torch.int32, device=dev)
+    work = None
+    if method == RS_PAIR_GJK:
+        work = torch.empty(max(1, n) * int(_lib.rs_pair_contact_work_floats()), dtype=torch.float32, device=dev)
+    _check(_lib.rs_pair_contact(dmodel.handle, n, ga_t.data_ptr(), gb_t.data_ptr(), pa.data_ptr(), pb.data_ptr(),
+                                method, lanes, work.data_ptr() if work is not None else None, out.data_ptr(),
+                                hit.data_ptr(), _stream_handle(stream)))
+    return hit, out
 
 
 def state_field_directory():

@@ -186,6 +186,19 @@ int rs_reset_caches(const rs_model* model, void* state, int n_envs, const int32_
 /* Bitwise OR of the per-environment error words into *out (synchronises the stream). */
 int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_t* out, void* stream);
 
+/* One contact detection per item, for tests and diagnostics (the detectors of the narrowphase on their own; the
+ * restatement's counterpart is the oracle's pair detection): geoms ga[i], gb[i] of the model at world poses
+ * pose_a[7 i .. 7 i + 6], pose_b[...] (x, y, z, qw, qx, qy, qz), with GJK/EPA (RS_PAIR_GJK, collider/gjk.py:170-446)
+ * or MPR without a warm-start direction (RS_PAIR_MPR, collider/mpr.py:769-818). lanes = 64 runs each item on a whole
+ * wavefront (the fused kernel's GJK/EPA: polytope in LDS, in work when it outgrows that); lanes = 1 runs one item per
+ * lane (the per-stage kernels' detectors). GJK/EPA needs work: n * rs_pair_contact_work_floats() floats. Results:
+ * out[7 i ..] = normal (from a to b), position, penetration; hit[i] = 1 contact, 0 none, -1 invalid item (geom
+ * index out of range, or a plane or terrain geom). Every pointer is a device pointer. */
+enum { RS_PAIR_GJK = 0, RS_PAIR_MPR = 1 };
+int64_t rs_pair_contact_work_floats(void);
+int rs_pair_contact(const rs_model* model, int n, const int32_t* ga, const int32_t* gb, const float* pose_a,
+                    const float* pose_b, int method, int lanes, float* work, float* out, int32_t* hit, void* stream);
+
 /* Launch configuration of the fused substep for one model: one 64-lane wavefront per environment,
  * envs_per_block wavefronts per workgroup (1..8), whether the environment's small fields are staged in LDS
  * (use_lds = 1, only when they fit) or in the env-major work field, and whether environments are started in order of

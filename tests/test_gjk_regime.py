@@ -1,16 +1,17 @@
-"""How often each workload meets the MPR -> GJK/EPA hand-over that is not built (narrowphase.py:729-734).
+"""How often each workload meets the MPR -> GJK/EPA hand-over (narrowphase.py:726-734).
 
 With the default MPR collision path the reference re-runs a detection with GJK/EPA when MPR reports a penetration
 deeper than the pair's tolerance and either no contact direction was cached for the pair or the overlap is large
-(mc_tolerance * penetration >= mpr_to_gjk_overlap_ratio * tolerance). This library keeps the MPR result there; the
-CPU restatement counts those detections (n_gjk_regime: first detections, n_gjk_regime_perturbed: the multi-contact
-perturbed ones), so the parity of a workload can be stated with the hand-over's reach measured:
+(mc_tolerance * penetration >= mpr_to_gjk_overlap_ratio * tolerance). Both the CPU restatement and the device kernels
+take the hand-over; the restatement counts those detections (n_gjk_regime: first detections,
+n_gjk_regime_perturbed: the multi-contact perturbed ones), which tells how much of each workload's parity rests on the
+GJK/EPA path:
 
-- the legged workload (sphere feet and capsule legs on a plane, analytic contacts) never reaches it: its parity does
-  not depend on the missing GJK/EPA path;
+- the legged workload (sphere feet and capsule legs on a plane, analytic contacts) never reaches it;
 - the box pile reaches it only on impact frames, for a small fraction of its contacts;
 - the Franka workload with joint angles drawn uniformly within the limits reaches it for a large share of its
-  contacts (deep self-penetration between hull geoms), so its contact set differs from the reference's there.
+  contacts (deep self-penetration between hull geoms); its device contact sets are compared with the restatement's
+  in the GPU stage-parity tests.
 """
 
 from __future__ import annotations
