@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -365,6 +366,41 @@ __global__ void __launch_bounds__(kBlock) k_pair_lane(DModel m, int n, const int
     pair_out(i, valid, col, nrm, pos, pen, out, hit);
 }
 
+// ---------------------------------------------------------------------- error-word reduction without allocation
+// rs_errno_reduce_noalloc keeps its partial results in `scratch`, a file-scope device array (each GPU holds its own
+// copy until the process ends): one partial OR per block of kErrnoBlock environments in words [0, n_blocks), the
+// final OR in word kErrnoMaxPartials. Every call rewrites every word it reads, so nothing has to start at zero; the
+// array keeps the last call's contents, and calls that overlap on one GPU race.
+constexpr int kErrnoBlock = 256;
+constexpr int kErrnoMaxPartials = 4096;  // at most kErrnoMaxPartials * kErrnoBlock = 1,048,576 environments per call
+static __device__ __align__(256) int32_t scratch[kErrnoMaxPartials + 64];
+
+__device__ __forceinline__ int32_t block_or(int32_t v, int32_t* w) {
+    for (int off = 1; off < kWave; off <<= 1) v |= __shfl_xor(v, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) w[threadIdx.x / kWave] = v;
+    __syncthreads();
+    int32_t r = 0;
+    for (int i = 0; i < kErrnoBlock / kWave; i++) r |= w[i];
+    return r;
+}
+
+// OR of block blockIdx.x's environments' error words into partials[blockIdx.x]
+__global__ void __launch_bounds__(kErrnoBlock) k_errno_partial(DState st, int32_t* partials) {
+    __shared__ int32_t w[kErrnoBlock / kWave];
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = block_or(b < st.B ? FV(st, errbits, b)[0] : 0, w);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// OR of n partial words into *out (one block)
+__global__ void __launch_bounds__(kErrnoBlock) k_errno_final(const int32_t* partials, int n, int32_t* out) {
+    __shared__ int32_t w[kErrnoBlock / kWave];
+    int32_t v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v |= partials[i];
+    const int32_t r = block_or(v, w);
+    if (threadIdx.x == 0) *out = r;
+}
+
 // ------------------------------------------------------------------------------------------------ launch helpers
 static inline dim3 env_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -702,6 +738,50 @@ int rs_pair_contact(const rs_model* m, int n, const int32_t* ga, const int32_t* 
         hipLaunchKernelGGL(k_pair_lane, env_grid(n), dim3(kBlock), 0, s, m->dm, n, ga, gb, pose_a, pose_b, method,
                            work, out, hit);
     HIP_OK(hipGetLastError());
+    return RS_OK;
+}
+
+// Device address of `scratch` on each device, taken once per device with that device current (the array's name in
+// host code is only a stand-in for the symbol, not a device address).
+static constexpr int kMaxDevices = 64;
+static int32_t* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_of_current_device(int32_t** out) {
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(RS_E_DEVICE, "rs_errno_reduce_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(RS_E_DEVICE, "rs_errno_reduce_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = (int32_t*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return RS_OK;
+}
+
+int rs_errno_reduce_noalloc(const rs_model* m, const void* state, int n_envs, int32_t* out, void* stream) {
+    if (!m || !state || n_envs < 1 || !out) return fail(RS_E_INVALID, "rs_errno_reduce_noalloc: bad argument");
+    const int64_t n_blocks = ((int64_t)n_envs + kErrnoBlock - 1) / kErrnoBlock;
+    if (n_blocks > kErrnoMaxPartials)
+        return fail(RS_E_INVALID, "rs_errno_reduce_noalloc: n_envs exceeds the static scratch (at most " +
+                                      std::to_string((int64_t)kErrnoMaxPartials * kErrnoBlock) + " environments)");
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != m->device) return fail(RS_E_INVALID, "rs_errno_reduce_noalloc: the model lives on another device");
+    int32_t* sp = nullptr;
+    const int r = scratch_of_current_device(&sp);
+    if (r != RS_OK) return r;
+    hipStream_t s = (hipStream_t)stream;
+    const DState st = make_state(m, (void*)state, n_envs);
+    hipLaunchKernelGGL(k_errno_partial, dim3((unsigned)n_blocks), dim3(kErrnoBlock), 0, s, st, sp);
+    hipLaunchKernelGGL(k_errno_final, dim3(1), dim3(kErrnoBlock), 0, s, (const int32_t*)sp, (int)n_blocks,
+                       sp + kErrnoMaxPartials);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, sp + kErrnoMaxPartials, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
     return RS_OK;
 }
 

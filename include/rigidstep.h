@@ -199,6 +199,13 @@ int64_t rs_pair_contact_work_floats(void);
 int rs_pair_contact(const rs_model* model, int n, const int32_t* ga, const int32_t* gb, const float* pose_a,
                     const float* pose_b, int method, int lanes, float* work, float* out, int32_t* hit, void* stream);
 
+/* rs_errno_reduce without any allocation. No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture of the surrounding loop, tight serving loops). Its partial results live in a
+ * static device array of the library (one per GPU, held until the process ends), which limits n_envs to 1,048,576
+ * per call (larger counts are rejected with RS_E_INVALID). Calls that overlap on one GPU share that array and race.
+ * The model must live on the current device. Synchronises the stream, like rs_errno_reduce. */
+int rs_errno_reduce_noalloc(const rs_model* model, const void* state, int n_envs, int32_t* out, void* stream);
+
 /* Launch configuration of the fused substep for one model: one 64-lane wavefront per environment,
  * envs_per_block wavefronts per workgroup (1..8), whether the environment's small fields are staged in LDS
  * (use_lds = 1, only when they fit) or in the env-major work field, and whether environments are started in order of
