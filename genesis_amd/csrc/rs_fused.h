@@ -168,7 +168,7 @@ __device__ inline void wave_broadphase(const DModel& m, const LState& loc, const
 // detection) item); the pair's lane then merges their results in detection order, as the sequential loop does.
 // mc: per-pair records (kMcRec floats per lane), the item map and the item results, in the work field. Detections
 // handed to GJK/EPA run on the whole wavefront when there is one (polytope in gjk_lds, LDS or nullptr; in gjk_mem when
-// it outgrows that) and one per lane when there are several (polytopes in the kGjkLaneSlots slots at gjk_mem).
+// it outgrows that) and one per lane when there are several (polytopes in the lane slots at gjk_mem).
 __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
                                         float* mc, float* gjk_lds, float* gjk_mem, int lane) {
     const int nb = FP(loc, n_broad)[0];
@@ -237,8 +237,8 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
                                                 cp, pen, want_gjk);
                         }
                         // the detections handed to GJK/EPA: a single one runs on the whole wavefront (polytope in
-                        // LDS); several run one per lane, kGjkLaneSlots at a time, each with its polytope in a lane
-                        // slot of the work field; the lane that owns a detection takes its result
+                        // LDS); several run one per lane, each with its polytope in the lane's slot of the work
+                        // field; the lane that owns a detection takes its result
                         uint64_t gm = wave_ballot(act && want_gjk);
                         if (gm && !(gm & (gm - 1))) {
                             const int l = __builtin_ctzll(gm);
@@ -248,24 +248,14 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
                             float gpen = 0.f;
                             const bool gcol = gjk_contact_wave(m, gp, lane, gjk_lds, gjk_mem, gn, gpos, gpen);
                             if (lane == l) col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
-                        } else {
-#pragma clang loop unroll(disable)
-                            while (gm) {
-                                uint64_t pass = 0;
-                                for (int k = 0; k < kGjkLaneSlots && gm; k++) {
-                                    pass |= gm & (~gm + 1);
-                                    gm &= gm - 1;
-                                }
-                                if ((pass >> lane) & 1) {
-                                    const int slot = __popcll(pass & ((1ull << lane) - 1));
-                                    const GjkPair gp = make_gjk_pair(m, dga, dgb, pa, qa, pb, qb);
-                                    f3 gn = zero3(), gpos = zero3();
-                                    float gpen = 0.f;
-                                    const bool gcol = gjk_contact_mem(m, gp, gjk_mem + (size_t)slot * kGjkMemWords, gn,
-                                                                      gpos, gpen);
-                                    col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
-                                }
-                            }
+                        } else if (gm) {
+                            const bool mine = (gm >> lane) & 1;
+                            const GjkPair gp = make_gjk_pair(m, dga, dgb, pa, qa, pb, qb);
+                            f3 gn = zero3(), gpos = zero3();
+                            float gpen = 0.f;
+                            bool gcol = false;
+                            gjk_contact_lanes(m, gp, mine, lane, gjk_mem, gcol, gn, gpos, gpen);
+                            if (mine) col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
                         }
                         if (round == 0) {
                             if (act) convex_first(m, loc, 0, st, b, ga, gb, col, n, cp, pen, sink, f);

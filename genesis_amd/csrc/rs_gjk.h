@@ -470,15 +470,74 @@ __device__ __forceinline__ f3 triangle_affine_coords(f3 p, f3 t1, f3 t2, f3 t3) 
 // position in the candidate map (-1: not a candidate, -2: deleted), the outward normal and the squared distance of the
 // face's plane from the origin.
 
-// Polytopes in memory per environment: the fused kernel runs up to this many GJK/EPA detections of one environment
-// side by side, one per lane (slot 0 also serves the per-stage kernels and the wavefront's fallback).
-constexpr int kGjkLaneSlots = 16;
 
 // Words of MemStore's slice of the work field (each 4 bytes).
 constexpr int kMemVertWords = 11;  // mk xyz, o1 xyz, o2 xyz, id1, id2
 constexpr int kMemFaceWords = 12;  // v1 v2 v3 a1 a2 a3 map nx ny nz d2 (pad)
 constexpr int kGjkMemWords = kEpaVerts * kMemVertWords + kEpaFaces * kMemFaceWords + kEpaFaces + 2 * kEpaHorizon +
                              2 * kEpaStack;
+
+// Sequential scans over a polytope store S (one lane per detection).
+// epa.py:989-1001: the candidate face nearest the origin (first in map order among equal distances), if its squared
+// distance is below lower2 (nearest is returned unchanged otherwise). The scan runs over the face records (a face is a
+// candidate when its map position is set) rather than through the map, so that the loads of different faces are
+// independent; the winner is the same: the smallest distance, and among equal distances the smallest map position.
+template <class S>
+__device__ __forceinline__ int nearest_face_scan(const S& st, int n_map, int n_faces, int nearest, float& lower2) {
+    const float lim = lower2;
+    float bd = lim;
+    int bi = 0x7fffffff, bf = -1;
+    auto take = [&](int f, int mi, float d2) {
+        if (mi >= 0 && mi < n_map && d2 < lim && (bf < 0 || d2 < bd || (d2 == bd && mi < bi))) {
+            bd = d2;
+            bi = mi;
+            bf = f;
+        }
+    };
+    int f = 0;
+    for (; f + 4 <= n_faces; f += 4) {
+        int mi[4];
+        float d2[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            mi[u] = st.fmapidx(f + u);
+            d2[u] = st.fdist2(f + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) take(f + u, mi[u], d2[u]);
+    }
+    for (; f < n_faces; f++) take(f, st.fmapidx(f), st.fdist2(f));
+    if (bf < 0) return nearest;
+    lower2 = bd;
+    return bf;
+}
+// epa.py:1335-1350: the orientation of the polytope's vertex farthest from the face's plane (other than its own)
+template <class S>
+__device__ __forceinline__ float max_orient_seq(const S& st, int nverts, int v1, int v2, int v3, f3 normal, f3 fc,
+                                                float init) {
+    float mo = init, ma = fabsf(init);
+#pragma unroll 4
+    for (int iv = 0; iv < nverts; iv++) {
+        if (iv != v1 && iv != v2 && iv != v3) {
+            const float o = dot3(normal, st.vmk(iv) - fc);
+            if (fabsf(o) > ma) {
+                ma = fabsf(o);
+                mo = o;
+            }
+        }
+    }
+    return mo;
+}
+// epa.py:1036-1046: another vertex with the same pair of support ids
+template <class S>
+__device__ __forceinline__ bool repeated_seq(const S& st, int nverts, int wi) {
+    const int a = st.vid1(wi), b = st.vid2(wi);
+    bool rep = false;
+#pragma unroll 4
+    for (int i = 0; i < nverts; i++)
+        if (i != wi && st.vid1(i) == a && st.vid2(i) == b) rep = true;
+    return rep;
+}
 
 // The polytope in memory (one lane per detection).
 struct MemStore {
@@ -535,58 +594,13 @@ struct MemStore {
 
     // epa.py:989-1001: the candidate face nearest the origin (first in map order among equal distances), if its
     // squared distance is below lower2; nearest is returned unchanged otherwise
-    // The scan runs over the face records (a face is a candidate when its map position is set) rather than through
-    // the map, so that the loads of different faces are independent; the winner is the same: the smallest distance,
-    // and among equal distances the smallest map position.
     __device__ __forceinline__ int nearest_face(int n_map, int n_faces, int nearest, float& lower2) const {
-        const float lim = lower2;
-        float bd = lim;
-        int bi = 0x7fffffff, bf = -1;
-        auto take = [&](int f, int mi, float d2) {
-            if (mi >= 0 && mi < n_map && d2 < lim && (bf < 0 || d2 < bd || (d2 == bd && mi < bi))) {
-                bd = d2;
-                bi = mi;
-                bf = f;
-            }
-        };
-        int f = 0;
-        for (; f + 4 <= n_faces; f += 4) {
-            int mi[4];
-            float d2[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                mi[u] = fmapidx(f + u);
-                d2[u] = fdist2(f + u);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) take(f + u, mi[u], d2[u]);
-        }
-        for (; f < n_faces; f++) take(f, fmapidx(f), fdist2(f));
-        if (bf < 0) return nearest;
-        lower2 = bd;
-        return bf;
+        return nearest_face_scan(*this, n_map, n_faces, nearest, lower2);
     }
-    // epa.py:1335-1350: the orientation of the polytope's vertex farthest from the face's plane (other than its own)
     __device__ __forceinline__ float max_orient(int nverts, int v1, int v2, int v3, f3 normal, f3 fc, float init) const {
-        float mo = init, ma = fabsf(init);
-        for (int iv = 0; iv < nverts; iv++) {
-            if (iv != v1 && iv != v2 && iv != v3) {
-                const float o = dot3(normal, vmk(iv) - fc);
-                if (fabsf(o) > ma) {
-                    ma = fabsf(o);
-                    mo = o;
-                }
-            }
-        }
-        return mo;
+        return max_orient_seq(*this, nverts, v1, v2, v3, normal, fc, init);
     }
-    // epa.py:1036-1046: another vertex with the same pair of support ids
-    __device__ __forceinline__ bool repeated(int nverts, int wi) const {
-        const int a = vid1(wi), b = vid2(wi);
-        for (int i = 0; i < nverts; i++)
-            if (i != wi && vid1(i) == a && vid2(i) == b) return true;
-        return false;
-    }
+    __device__ __forceinline__ bool repeated(int nverts, int wi) const { return repeated_seq(*this, nverts, wi); }
 };
 
 // ------------------------------------------------------------------------------------- wavefront reductions
@@ -642,11 +656,12 @@ constexpr int kGjkLdsWords = kLdsVerts * kLdsVertWords + kLdsFaces * kLdsFaceWor
                              kLdsStack / 2;
 static_assert(kLdsVerts <= kWave && kLdsFaces <= 2 * kWave && kLdsFaces <= 254, "LDS polytope packing");
 
-struct LdsStore {
-    static constexpr int kTeam = kWave;
+template <bool kWaveOps>
+struct CompactStore {
+    static constexpr int kTeam = kWaveOps ? kWave : 1;
     static constexpr bool kAbort = true;
     static constexpr int kFaces = kLdsFaces, kVerts = kLdsVerts, kHorizon = kLdsHorizon, kStack = kLdsStack;
-    float* w;  // kGjkLdsWords words of LDS, 16-byte aligned
+    float* w;  // kGjkLdsWords words, 16-byte aligned
     int lane;
 
     __device__ __forceinline__ float* vrec(int i) const { return w + i * kLdsVertWords; }
@@ -717,6 +732,7 @@ struct LdsStore {
     // map order): each lane takes faces lane and lane + 64, then two reductions (the distance, then the position
     // among the faces at that distance)
     __device__ __forceinline__ int nearest_face(int n_map, int n_faces, int nearest, float& lower2) const {
+        if constexpr (!kWaveOps) return nearest_face_seq(n_map, n_faces, nearest, lower2);
         const float inf = __int_as_float(0x7f800000);
         float bd = inf;
         int bi = 0x7fffffff, bf = -1;
@@ -742,6 +758,7 @@ struct LdsStore {
     }
     // epa.py:1335-1350 with one vertex per lane: the first vertex (other than the face's) farthest from the plane
     __device__ __forceinline__ float max_orient(int nverts, int v1, int v2, int v3, f3 normal, f3 fc, float init) const {
+        if constexpr (!kWaveOps) return max_orient_seq(*this, nverts, v1, v2, v3, normal, fc, init);
         const float ma = fabsf(init);
         float key = -1.f, o = 0.f;
         if (lane < nverts && lane != v1 && lane != v2 && lane != v3) {
@@ -753,10 +770,17 @@ struct LdsStore {
         return rl_f(o, __builtin_ctzll(wave_ballot(key == mk)));
     }
     __device__ __forceinline__ bool repeated(int nverts, int wi) const {
+        if constexpr (!kWaveOps) return repeated_seq(*this, nverts, wi);
         const int a = vid1(wi), b = vid2(wi);
         return wave_ballot(lane < nverts && lane != wi && vid1(lane) == a && vid2(lane) == b) != 0;
     }
+    // one lane alone: the face records scanned with independent loads (as in MemStore)
+    __device__ __forceinline__ int nearest_face_seq(int n_map, int n_faces, int nearest, float& lower2) const {
+        return nearest_face_scan(*this, n_map, n_faces, nearest, lower2);
+    }
 };
+using LdsStore = CompactStore<true>;    // one detection per wavefront, the polytope in the wavefront's LDS
+using LaneStore = CompactStore<false>;  // one detection per lane, the polytope in the lane's slot of the work field
 
 // -------------------------------------------------------------------------------------------- the detection
 // The detection's state; the model and the pair are passed to every step (a struct holding references to them would
@@ -1171,6 +1195,20 @@ struct GjkEpa {
     }
 };
 
+// An environment's GJK/EPA area of the work field: one polytope at the reference's capacities (the per-stage kernels'
+// and the fallback of the compact ones), then one compact polytope per lane of the fused kernel's wavefront.
+constexpr int kGjkMemSlotWords = (kGjkMemWords + 3) / 4 * 4;
+constexpr int kGjkWorkWords = kGjkMemSlotWords + kWave * kGjkLdsWords;
+__device__ __forceinline__ float* gjk_lane_slot(float* gjk_mem, int lane) {
+    return gjk_mem + kGjkMemSlotWords + (size_t)lane * kGjkLdsWords;
+}
+
+// One detection per lane of a wavefront (the lanes with want set), each with its polytope in the lane's compact slot of
+// gjk_mem; a detection that outgrows its slot runs again afterwards on the whole wavefront with the polytope at the
+// reference's capacities. Each wanting lane's outputs are set; the others' are left alone.
+__device__ __forceinline__ void gjk_contact_lanes(const DModel& m, const GjkPair& p, bool want, int lane,
+                                                  float* gjk_mem, bool& col, f3& normal, f3& pos, float& pen);
+
 // One detection by one lane (or by every lane of a wavefront with the same inputs), the polytope in memory (w:
 // kGjkMemWords words of the environment's work slice) at the reference's capacities.
 __device__ __forceinline__ bool gjk_contact_mem(const DModel& m, const GjkPair& p, float* w, f3& normal, f3& pos,
@@ -1196,6 +1234,41 @@ __device__ __forceinline__ bool gjk_contact_wave(const DModel& m, const GjkPair&
         if (!g.overflow) return col;
     }
     return gjk_contact_mem(m, p, mem, normal, pos, pen);
+}
+
+__device__ __forceinline__ void gjk_contact_lanes(const DModel& m, const GjkPair& p, bool want, int lane,
+                                                  float* gjk_mem, bool& col, f3& normal, f3& pos, float& pen) {
+    bool over = false;
+    if (want) {
+        GjkEpa<LaneStore> g;
+        g.S.w = gjk_lane_slot(gjk_mem, lane);
+        g.S.lane = lane;
+        g.lane = lane;
+        col = g.contact(m, p, normal, pos, pen);
+        over = g.overflow;
+    }
+#pragma clang loop unroll(disable)
+    for (uint64_t om = wave_ballot(over); om; om &= om - 1) {
+        const int l = __builtin_ctzll(om);
+        GjkPair q;
+        q.ga = rl_i(p.ga, l);
+        q.gb = rl_i(p.gb, l);
+        q.pa = rl3(p.pa, l);
+        q.pb = rl3(p.pb, l);
+        q.qa = rl4(p.qa, l);
+        q.qb = rl4(p.qb, l);
+        q.ia = ginfo(m, q.ga);
+        q.ib = ginfo(m, q.gb);
+        f3 qn = zero3(), qp = zero3();
+        float qpen = 0.f;
+        const bool qc = gjk_contact_mem(m, q, gjk_mem, qn, qp, qpen);
+        if (lane == l) {
+            col = qc;
+            normal = qn;
+            pos = qp;
+            pen = qpen;
+        }
+    }
 }
 
 // narrowphase.py:726-734: after MPR, a deep penetration with no cached direction, or a large overlap, is detected

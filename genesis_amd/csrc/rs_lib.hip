@@ -149,9 +149,9 @@ static void fused_layout(const rs_model_header& h, FusedArgs* fa) {
     fa->off_tmp = (int)(fa->off_rowf + RW_COUNT * up4(C));
     fa->off_pc = (int)(fa->off_tmp + kTmpArrays * up4(C));
     fa->off_mc = (int)(fa->off_pc + up4((int64_t)kWave * (h.n_contacts_per_pair > 0 ? h.n_contacts_per_pair : 1) * 8));
-    // GJK/EPA polytopes: kGjkLaneSlots slots (the per-stage kernels use the first)
+    // GJK/EPA polytopes: one at the reference's capacities (the per-stage kernels use it), one compact one per lane
     fa->off_gjk = (int)up4(fa->off_mc + kMcFloats);
-    int64_t total = fa->off_gjk + (int64_t)kGjkLaneSlots * kGjkMemWords;
+    int64_t total = fa->off_gjk + (int64_t)kGjkWorkWords;
     fa->work_stride = (total + 63) / 64 * 64;
 }
 
@@ -328,7 +328,10 @@ __device__ __forceinline__ void pair_out(int i, bool valid, bool col, f3 nrm, f3
     hit[i] = valid ? (col ? 1 : 0) : -1;
 }
 
-// one wavefront per item: the fused kernel's GJK/EPA (polytope in the lanes' registers)
+// per-item floats of rs_pair_contact's work buffer: a polytope at the reference's capacities plus a compact one
+static constexpr int kPairWorkWords = kGjkMemSlotWords + kGjkLdsWords;
+
+// one wavefront per item: the fused kernel's GJK/EPA for a single detection (polytope in LDS)
 __global__ void __launch_bounds__(kWave) k_pair_gjk_wave(DModel m, int n, const int32_t* ga, const int32_t* gb,
                                                          const float* pa, const float* pb, float* work, float* out,
                                                          int32_t* hit) {
@@ -340,12 +343,13 @@ __global__ void __launch_bounds__(kWave) k_pair_gjk_wave(DModel m, int n, const 
     f3 nrm = zero3(), pos = zero3();
     float pen = 0.f;
     const bool col =
-        valid ? gjk_contact_wave(m, p, lane_id(), lds, work + (size_t)i * kGjkMemWords, nrm, pos, pen) : false;
+        valid ? gjk_contact_wave(m, p, lane_id(), lds, work + (size_t)i * kPairWorkWords, nrm, pos, pen) : false;
     if (lane_id() == 0) pair_out(i, valid, col, nrm, pos, pen, out, hit);
 }
 
-// one lane per item: the per-stage kernels' GJK/EPA (polytope in work, kGjkMemWords floats per item) or MPR without a
-// warm-start direction
+// one lane per item: the fused kernel's lane-parallel GJK/EPA (compact polytope per lane, the block's full polytope for
+// detections that outgrow it; the block of 64 items owns work[64 * blockIdx.x * kPairWorkWords ...]) or MPR without
+// a warm-start direction
 __global__ void __launch_bounds__(kBlock) k_pair_lane(DModel m, int n, const int32_t* ga, const int32_t* gb,
                                                       const float* pa, const float* pb, int method, float* work,
                                                       float* out, int32_t* hit) {
@@ -356,12 +360,11 @@ __global__ void __launch_bounds__(kBlock) k_pair_lane(DModel m, int n, const int
     f3 nrm = zero3(), pos = zero3();
     float pen = 0.f;
     bool col = false;
-    if (valid) {
-        if (method == RS_PAIR_GJK) {
-            col = gjk_contact_mem(m, p, work + (size_t)i * kGjkMemWords, nrm, pos, pen);
-        } else {
-            mpr_contact(m, p.ga, p.gb, zero3(), p.pa, p.qa, p.pb, p.qb, col, nrm, pen, pos);
-        }
+    if (method == RS_PAIR_GJK) {
+        gjk_contact_lanes(m, p, valid, lane_id(), work + (size_t)blockIdx.x * kBlock * kPairWorkWords, col, nrm, pos,
+                          pen);
+    } else if (valid) {
+        mpr_contact(m, p.ga, p.gb, zero3(), p.pa, p.qa, p.pb, p.qb, col, nrm, pen, pos);
     }
     pair_out(i, valid, col, nrm, pos, pen, out, hit);
 }
@@ -718,7 +721,7 @@ int rs_errno_reduce(const rs_model* m, const void* state, int n_envs, int32_t* o
     return RS_OK;
 }
 
-int64_t rs_pair_contact_work_floats(void) { return kGjkMemWords; }
+int64_t rs_pair_contact_work_floats(void) { return kPairWorkWords; }
 
 int rs_pair_contact(const rs_model* m, int n, const int32_t* ga, const int32_t* gb, const float* pose_a,
                     const float* pose_b, int method, int lanes, float* work, float* out, int32_t* hit, void* stream) {

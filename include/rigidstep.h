@@ -190,8 +190,10 @@ int rs_errno_reduce(const rs_model* model, const void* state, int n_envs, int32_
  * restatement's counterpart is the oracle's pair detection): geoms ga[i], gb[i] of the model at world poses
  * pose_a[7 i .. 7 i + 6], pose_b[...] (x, y, z, qw, qx, qy, qz), with GJK/EPA (RS_PAIR_GJK, collider/gjk.py:170-446)
  * or MPR without a warm-start direction (RS_PAIR_MPR, collider/mpr.py:769-818). lanes = 64 runs each item on a whole
- * wavefront (the fused kernel's GJK/EPA: polytope in LDS, in work when it outgrows that); lanes = 1 runs one item per
- * lane (the per-stage kernels' detectors). GJK/EPA needs work: n * rs_pair_contact_work_floats() floats. Results:
+ * wavefront (the fused kernel's GJK/EPA for a single detection: polytope in LDS, in work when it outgrows that);
+ * lanes = 1 runs one item per lane (GJK/EPA as the fused kernel runs several detections of an environment: a compact
+ * polytope per lane in work; MPR as the per-stage kernels run it). GJK/EPA needs work: n *
+ * rs_pair_contact_work_floats() floats. Results:
  * out[7 i ..] = normal (from a to b), position, penetration; hit[i] = 1 contact, 0 none, -1 invalid item (geom
  * index out of range, or a plane or terrain geom). Every pointer is a device pointer. */
 enum { RS_PAIR_GJK = 0, RS_PAIR_MPR = 1 };
