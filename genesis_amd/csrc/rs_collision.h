@@ -146,6 +146,7 @@ __device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
     float dmax = -1e20f;
     f3 v = zero3();
     const int start = m.geom_support_start[g];
+    const int vstart = m.geom_vert_start[g];
     for (int i4 = 0; i4 < 4; i4++) {
         // the reference's cell % res; theta lies in [-pi, pi], so the cell lies in [0, res] and the modulo is one
         // conditional subtraction (a NaN cell stays NaN, as with fmodf)
@@ -162,8 +163,9 @@ __device__ inline f3 support_mesh(const DModel& m, f3 d, int g) {
         }
         // NaN directions map to cell (0, 1); keep the index inside the table in every case
         i = min(max(i, 0), res - 1);
-        cfptr pp = &m.support_v[3 * (size_t)(start + i * res + j)];
-        f3 p = mk3(pp[0], pp[1], pp[2]);
+        // the table cell holds the id of its vertex; the vertex itself comes from the (small) vertex array, so the
+        // lookup touches 4 bytes of the large table instead of 12
+        const f3 p = mvec3(m.verts, vstart + m.support_vid[(size_t)(start + i * res + j)]);
         float dt = dot3(p, d);
         if (dt > dmax) {
             v = p;

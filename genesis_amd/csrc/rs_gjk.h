@@ -71,8 +71,8 @@ __device__ __forceinline__ GInfo ginfo(const DModel& m, int g) {
 }
 
 // support_field.py:139-181 _func_support_mesh with the id of the chosen vertex (support_vid); start: the geom's
-// support-table start
-__device__ __forceinline__ f3 support_mesh_id(const DModel& m, f3 d, int start, int& vid) {
+// support-table start, vstart: its first vertex (a cell's point is the vertex whose id the cell holds)
+__device__ __forceinline__ f3 support_mesh_id(const DModel& m, f3 d, int start, int vstart, int& vid) {
     const int res = 180;
     const float pi = 3.14159265358979323846f;
     const float theta = atan2f(d.y, d.x);
@@ -95,14 +95,13 @@ __device__ __forceinline__ f3 support_mesh_id(const DModel& m, f3 d, int start, 
             if (j == 0) j = 1;
         }
         i = min(max(i, 0), res - 1);
-        const size_t idx = (size_t)(start + i * res + j);
-        cfptr pp = &m.support_v[3 * idx];
-        const f3 p = mk3(pp[0], pp[1], pp[2]);
+        const int id = m.support_vid[(size_t)(start + i * res + j)];
+        const f3 p = mvec3(m.verts, vstart + id);
         const float dt = dot3(p, d);
         if (dt > dmax) {
             v = p;
             dmax = dt;
-            vid = m.support_vid[idx];
+            vid = id;
         }
     }
     return v;
@@ -136,7 +135,7 @@ __device__ __forceinline__ f3 gjk_support_geom(const DModel& m, f3 d, const GInf
     }
     if (gi.sup < 0) return pos;
     const f3 dm = qrot(d, qinv(quat));
-    return tr_point(support_mesh_id(m, dm, gi.sup, vid), pos, quat);
+    return tr_point(support_mesh_id(m, dm, gi.sup, gi.vstart, vid), pos, quat);
 }
 
 // support_field.py:312-395: a box has 2^(zero components of the local direction) support points; the mesh count
@@ -243,9 +242,8 @@ __device__ __forceinline__ SVert support_pair_wave(const DModel& m, const GjkPai
             if (j == 0) j = 1;
         }
         i = min(max(i, 0), res - 1);
-        const size_t idx = (size_t)(gi.sup + i * res + j);
-        cfptr pp = &m.support_v[3 * idx];
-        const f3 v = mk3(pp[0], pp[1], pp[2]);
+        const int id = m.support_vid[(size_t)(gi.sup + i * res + j)];
+        const f3 v = mvec3(m.verts, gi.vstart + id);
         const float dt = dot3(v, dm);
         const bool ok = dt > -1e20f;
         const float key = ok ? dt : -__int_as_float(0x7f800000);
@@ -258,7 +256,7 @@ __device__ __forceinline__ SVert support_pair_wave(const DModel& m, const GjkPai
         const bool found = (win & 0xf) != 0;
         src = found ? base + __builtin_ctzll(win & 0xf) : base;
         sp = tr_point(found ? v : zero3(), pos, quat);
-        vid = found ? m.support_vid[idx] : 0;
+        vid = found ? id : 0;
     } else {
         sp = gjk_support_geom(m, dir, gi, pos, quat, vid);
     }
@@ -651,7 +649,7 @@ __device__ __forceinline__ int wave_min_i(int x) {
 // writes are made by every lane with the same value. The scans run one entry per lane and reduce with DPP.
 constexpr int kLdsFaces = 128, kLdsVerts = 32, kLdsHorizon = 32, kLdsStack = 256;
 constexpr int kLdsVertWords = 12;  // mk xyz, o1 xyz, o2 xyz, id1, id2, pad (16-byte aligned)
-constexpr int kLdsFaceWords = 8;   // [v1 | v2 << 8 | v3 << 16 | (map + 2) << 24], [a1 | a2 << 10 | a3 << 20], n xyz, d2
+constexpr int kLdsFaceWords = 8;   // vertices and map position (bytes), adjacent faces (halfwords), normal, d2
 constexpr int kGjkLdsWords = kLdsVerts * kLdsVertWords + kLdsFaces * kLdsFaceWords + kLdsFaces / 4 + kLdsHorizon / 2 +
                              kLdsStack / 2;
 static_assert(kLdsVerts <= kWave && kLdsFaces <= 2 * kWave && kLdsFaces <= 254, "LDS polytope packing");
@@ -684,11 +682,19 @@ struct CompactStore {
     __device__ __forceinline__ int vid1(int i) const { return __float_as_int(vrec(i)[9]); }
     __device__ __forceinline__ int vid2(int i) const { return __float_as_int(vrec(i)[10]); }
 
+    // face record: bytes 0-2 the vertices, byte 3 the map position + 2, bytes 4-9 the adjacent faces (16 bits each),
+    // words 3-5 the normal, word 6 the squared distance; every field has its own byte, halfword or word, so that a
+    // field is written without reading the record back
     __device__ __forceinline__ int* fw(int f) const { return (int*)frec(f); }
+    __device__ __forceinline__ uint8_t* fb(int f) const { return (uint8_t*)frec(f); }
+    __device__ __forceinline__ uint16_t* fa(int f) const { return (uint16_t*)(frec(f) + 1); }
+    // a new face starts outside the candidate map (map position -1, what a successful attach sets; a failed attach
+    // ends the EPA before the position is read)
     __device__ __forceinline__ void set_face(int f, int v1, int v2, int v3, int a1, int a2, int a3) const {
         int* r = fw(f);
-        r[0] = (r[0] & (int)0xff000000) | v1 | (v2 << 8) | (v3 << 16);
-        r[1] = a1 | (a2 << 10) | (a3 << 20);
+        r[0] = v1 | (v2 << 8) | (v3 << 16) | (1 << 24);
+        r[1] = a1 | (a2 << 16);
+        r[2] = a3;
     }
     __device__ __forceinline__ void fverts(int f, int& v1, int& v2, int& v3) const {
         const int p = fw(f)[0];
@@ -696,22 +702,16 @@ struct CompactStore {
         v2 = (p >> 8) & 0xff;
         v3 = (p >> 16) & 0xff;
     }
-    __device__ __forceinline__ int fvert(int f, int k) const { return (fw(f)[0] >> (8 * k)) & 0xff; }
-    __device__ __forceinline__ int fadj(int f, int k) const { return (fw(f)[1] >> (10 * k)) & 0x3ff; }
-    __device__ __forceinline__ void set_fadj(int f, int k, int a) const {
-        int* r = fw(f);
-        r[1] = (r[1] & ~(0x3ff << (10 * k))) | (a << (10 * k));
-    }
-    __device__ __forceinline__ int fmapidx(int f) const { return ((uint32_t)fw(f)[0] >> 24) - 2; }
-    __device__ __forceinline__ void set_fmapidx(int f, int v) const {
-        int* r = fw(f);
-        r[0] = (r[0] & 0x00ffffff) | ((v + 2) << 24);
-    }
-    __device__ __forceinline__ f3 fnormal(int f) const { const float* r = frec(f); return mk3(r[2], r[3], r[4]); }
-    __device__ __forceinline__ float fdist2(int f) const { return frec(f)[5]; }
+    __device__ __forceinline__ int fvert(int f, int k) const { return fb(f)[k]; }
+    __device__ __forceinline__ int fadj(int f, int k) const { return fa(f)[k]; }
+    __device__ __forceinline__ void set_fadj(int f, int k, int a) const { fa(f)[k] = (uint16_t)a; }
+    __device__ __forceinline__ int fmapidx(int f) const { return (int)fb(f)[3] - 2; }
+    __device__ __forceinline__ void set_fmapidx(int f, int v) const { fb(f)[3] = (uint8_t)(v + 2); }
+    __device__ __forceinline__ f3 fnormal(int f) const { const float* r = frec(f); return mk3(r[3], r[4], r[5]); }
+    __device__ __forceinline__ float fdist2(int f) const { return frec(f)[6]; }
     __device__ __forceinline__ void set_fgeom(int f, f3 n, float d2) const {
         float* r = frec(f);
-        r[2] = n.x; r[3] = n.y; r[4] = n.z; r[5] = d2;
+        r[3] = n.x; r[4] = n.y; r[5] = n.z; r[6] = d2;
     }
     __device__ __forceinline__ int map_at(int i) const { return fmap()[i]; }
     __device__ __forceinline__ void set_map_at(int i, int f) const { fmap()[i] = (uint8_t)f; }
