@@ -454,6 +454,13 @@ struct PlaneSink {
         int ip = SF(n_contacts, 0) - 1 - back;
         return ip >= 0 && norm3(p - LD3(contact_pos, ip)) < tol;
     }
+    __device__ int count() const { return SF(n_contacts, 0); }
+    // true when contact i exists and every coordinate of it lies within eps of p
+    __device__ bool same(int i, f3 p, float eps) const {
+        if (i >= SF(n_contacts, 0)) return false;
+        const f3 q = LD3(contact_pos, i);
+        return fabsf(p.x - q.x) < eps && fabsf(p.y - q.y) < eps && fabsf(p.z - q.z) < eps;
+    }
 };
 
 struct SlotSink {
@@ -474,6 +481,12 @@ struct SlotSink {
         if (ip < 0) return false;
         const float* s = slot + 8 * ip;
         return norm3(p - mk3(s[3], s[4], s[5])) < tol;
+    }
+    __device__ int count() const { return n; }
+    __device__ bool same(int i, f3 p, float eps) const {
+        if (i >= n) return false;
+        const float* s = slot + 8 * i;
+        return fabsf(p.x - s[3]) < eps && fabsf(p.y - s[4]) < eps && fabsf(p.z - s[5]) < eps;
     }
 };
 
@@ -796,6 +809,371 @@ __device__ inline void plane_box(const DModel& m, const S& st, int b, int ga, in
     }
 }
 
+// ---------------------------------------------------------------------------------------------------- box-box
+// box_contact.py:100-785 func_box_box_contact: separating-axis test over the 15 axes of two boxes, then the deepest
+// feature clipped against the reference face (face contact) or the two edges clipped against each other's faces
+// (edge contact). Candidate points go straight to the sink in the order the reference collects them; the reference's
+// dedup (every coordinate within EPS of a contact this pair already added) and its n_contacts_per_pair limit apply as
+// they are emitted. Components are picked with selects (no dynamically indexed private arrays).
+__device__ __forceinline__ float mel(const m33& A, int i, int j) {
+    const f3 r = i == 0 ? mk3(A.a[0][0], A.a[0][1], A.a[0][2])
+                        : (i == 1 ? mk3(A.a[1][0], A.a[1][1], A.a[1][2]) : mk3(A.a[2][0], A.a[2][1], A.a[2][2]));
+    return comp(r, j);
+}
+__device__ __forceinline__ f3 mrow3(const m33& A, int i) {
+    return i == 0 ? mk3(A.a[0][0], A.a[0][1], A.a[0][2])
+                  : (i == 1 ? mk3(A.a[1][0], A.a[1][1], A.a[1][2]) : mk3(A.a[2][0], A.a[2][1], A.a[2][2]));
+}
+// contact.py:16-31 rotaxis / rotmatx
+__device__ __forceinline__ f3 rotaxis3(f3 v, int i0, int i1, int i2, float f0, float f1, float f2) {
+    return mk3(comp(v, i0) * f0, comp(v, i1) * f1, comp(v, i2) * f2);
+}
+__device__ __forceinline__ m33 rotmatx3(const m33& A, int i0, int i1, int i2, float f0, float f1, float f2) {
+    const f3 r0 = mrow3(A, i0) * f0, r1 = mrow3(A, i1) * f1, r2 = mrow3(A, i2) * f2;
+    return m33{{{r0.x, r0.y, r0.z}, {r1.x, r1.y, r1.z}, {r2.x, r2.y, r2.z}}};
+}
+// box_contact.py:241-303 / :488-520: the frame that puts face q (0..5) of the reference box on +z
+__device__ __forceinline__ void box_face_frame(int q, m33& R, int& i0, int& i1, int& i2, float& f0, float& f1, float& f2) {
+    R = m33{{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}};
+    i0 = 0, i1 = 1, i2 = 2;
+    f0 = f1 = f2 = 1.f;
+    if (q == 0) R.a[0][2] = -1.f, R.a[1][1] = 1.f, R.a[2][0] = 1.f, i0 = 2, f0 = -1.f, i2 = 0;
+    else if (q == 1) R.a[0][0] = 1.f, R.a[1][2] = -1.f, R.a[2][1] = 1.f, i1 = 2, f1 = -1.f, i2 = 1;
+    else if (q == 2) R.a[0][0] = 1.f, R.a[1][1] = 1.f, R.a[2][2] = 1.f;
+    else if (q == 3) R.a[0][2] = 1.f, R.a[1][1] = 1.f, R.a[2][0] = -1.f, i0 = 2, i2 = 0, f2 = -1.f;
+    else if (q == 4) R.a[0][0] = 1.f, R.a[1][2] = 1.f, R.a[2][1] = -1.f, i1 = 2, i2 = 1, f2 = -1.f;
+    else R.a[0][0] = -1.f, R.a[1][1] = 1.f, R.a[2][2] = -1.f, f0 = -1.f, f2 = -1.f;
+}
+
+// adds one candidate (local point with its height already in place, distance dist) as the reference's add loop does
+template <class Sink>
+struct BoxEmit {
+    Sink& sink;
+    int ga, gb, pair, start, added, cap;
+    f3 base, nrm;
+    m33 R;
+    float hz, eps;
+    __device__ __forceinline__ void operator()(f3 pt, float dist) {
+        if (added >= cap) return;
+        pt.z = pt.z + hz;
+        const f3 cpos = base + mv33(R, pt);
+        bool valid = true;
+        for (int j = 0; j < added; j++)
+            if (sink.same(start + j, cpos, eps)) valid = false;
+        if (valid) {
+            sink.add(ga, gb, nrm, cpos, -dist, pair);
+            added++;
+        }
+    }
+};
+
+template <class Sink, class S>
+__device__ inline void box_box(const DModel& m, const S& st, int b, int ga, int gb, Sink& sink) {
+    const float EPS = m.eps, margin = 0.f, margin2 = margin * margin;
+    const int pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
+    const f3 pos1 = LD3(geoms_pos, ga), pos2 = LD3(geoms_pos, gb);
+    const f3 size1 = mk3(m.geom_data[7 * ga] / 2.f, m.geom_data[7 * ga + 1] / 2.f, m.geom_data[7 * ga + 2] / 2.f);
+    const f3 size2 = mk3(m.geom_data[7 * gb] / 2.f, m.geom_data[7 * gb + 1] / 2.f, m.geom_data[7 * gb + 2] / 2.f);
+    const m33 mat1 = quat_R(LD4(geoms_quat, ga), EPS), mat2 = quat_R(LD4(geoms_quat, gb), EPS);
+    const f3 pos21 = mv33(tr33(mat1), pos2 - pos1);
+    const f3 pos12 = mv33(tr33(mat2), pos1 - pos2);
+    const m33 rot = mm33(tr33(mat1), mat2), rott = tr33(rot);
+    m33 rotabs, rottabs;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) rotabs.a[i][j] = fabsf(rot.a[i][j]), rottabs.a[i][j] = fabsf(rott.a[i][j]);
+    const f3 plen2 = mv33(rotabs, size2);
+    const f3 plen1 = mv33(tr33(rotabs), size1);
+    float penetration = margin;
+#pragma unroll
+    for (int i = 0; i < 3; i++) penetration = penetration + size1[i] * 3.f + size2[i] * 3.f;
+    int code = -1, cle1 = 0, cle2 = 0;
+    bool is_return = false, in_ = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float c1 = -fabsf(pos21[i]) + size1[i] + plen2[i];
+        const float c2 = -fabsf(pos12[i]) + size2[i] + plen1[i];
+        if (c1 < -margin || c2 < -margin) is_return = true;
+        if (c1 < penetration) penetration = c1, code = i + 3 * (pos21[i] < 0.f ? 1 : 0);
+        if (c2 < penetration) penetration = c2, code = i + 3 * (pos12[i] < 0.f ? 1 : 0) + 6;
+    }
+    f3 clnorm = zero3();
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float rj0 = rott.a[j][0], rj1 = rott.a[j][1], rj2 = rott.a[j][2];
+            f3 t = i == 0 ? mk3(0.f, -rj2, rj1) : (i == 1 ? mk3(rj2, 0.f, -rj0) : mk3(-rj1, rj0, 0.f));
+            const float c1 = norm3(t);
+            t = mk3(t.x / c1, t.y / c1, t.z / c1);
+            if (c1 >= EPS) {
+                const float c2 = dot3(pos21, t);
+                float c3 = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    if (k != i) c3 = c3 + size1[k] * fabsf(t[k]);
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    if (k != j) c3 = c3 + size2[k] * rotabs.a[i][3 - k - j] / c1;
+                c3 = c3 - fabsf(c2);
+                if (c3 < -margin) is_return = true;
+                if (c3 < penetration * (1.f - 1e-12f)) {
+                    penetration = c3;
+                    cle1 = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        if (k != i && ((t[k] > 0.f) != (c2 < 0.f))) cle1 += 1 << k;
+                    cle2 = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        if (k != j && (((rot.a[i][3 - k - j] > 0.f) != (c2 < 0.f)) != ((k - j + 3) % 3 == 1)))
+                            cle2 += 1 << k;
+                    code = 12 + i * 3 + j;
+                    clnorm = t;
+                    in_ = c2 < 0.f;
+                }
+            }
+        }
+    }
+    if (code == -1 || is_return) return;
+    m33 rotmore;
+    int i0, i1, i2;
+    float f0, f1, f2;
+    BoxEmit<Sink> emit{sink, ga, gb, pair, sink.count(), 0, m.NCPP, zero3(), zero3(), rotmore, 0.f, EPS};
+    if (code < 12) {
+        // face contact (box_contact.py:241-457)
+        const int q1 = code % 6, q2 = code / 6;
+        box_face_frame(q1, rotmore, i0, i1, i2, f0, f1, f2);
+        m33 r;
+        f3 p, t1, sz;
+        if (q2) {
+            r = mm33(rotmore, tr33(rot));
+            p = rotaxis3(pos12, i0, i1, i2, f0, f1, f2);
+            t1 = rotaxis3(size2, i0, i1, i2, f0, f1, f2);
+            sz = size1;
+        } else {
+            r = rotmatx3(rot, i0, i1, i2, f0, f1, f2);
+            p = rotaxis3(pos21, i0, i1, i2, f0, f1, f2);
+            t1 = rotaxis3(size1, i0, i1, i2, f0, f1, f2);
+            sz = size2;
+        }
+        const m33 rt = tr33(r);
+        const f3 ss = mk3(fabsf(t1.x), fabsf(t1.y), fabsf(t1.z));
+        const float lx = ss.x, ly = ss.y, hz = ss.z;
+        p.z = p.z - hz;
+        int clcorner = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if (r.a[2][i] < 0.f) clcorner += 1 << i;
+        f3 lp = p;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            lp = lp + mk3(rt.a[i][0], rt.a[i][1], rt.a[i][2]) * sz[i] * ((clcorner & (1 << i)) ? 1.f : -1.f);
+        const f3 pts0 = lp;
+        f3 pts1 = zero3(), pts2 = zero3();
+        int mm = 1;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if (fabsf(r.a[2][i]) < 0.5f) {
+                const f3 e = mk3(rt.a[i][0], rt.a[i][1], rt.a[i][2]) * sz[i] * ((clcorner & (1 << i)) ? -2.f : 2.f);
+                if (mm == 1) pts1 = e;
+                else pts2 = e;
+                mm++;
+            }
+        const f3 pts3 = pts0 + pts1, pts4 = pts0 + pts2, pts5 = pts3 + pts2;
+        const m33 rr = mm33(q2 ? mat2 : mat1, tr33(rotmore));
+        const float sg = q2 ? -1.f : 1.f;
+        emit.base = q2 ? pos2 : pos1;
+        emit.nrm = -mk3(sg * rr.a[0][2], sg * rr.a[1][2], sg * rr.a[2][2]);
+        emit.R = rr;
+        emit.hz = hz;
+        // a candidate at or below the face: its distance is its height, halved (box_contact.py:414-434)
+        auto cand = [&](f3 pt) {
+            if (pt.z <= margin) {
+                pt.z = pt.z * 0.5f;
+                emit(pt, pt.z);
+            }
+        };
+        auto clip = [&](f3 A, f3 D) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const float a = A[q], bb = D[q], c = A[1 - q], d = D[1 - q];
+                if (fabsf(bb) > EPS) {
+#pragma unroll
+                    for (int jj = 0; jj < 2; jj++) {
+                        const float l = ss[q] * (float)(2 * jj - 1);
+                        const float c1 = (l - a) / bb;
+                        if (0.f <= c1 && c1 <= 1.f) {
+                            const float c2 = c + d * c1;
+                            if (fabsf(c2) <= ss[1 - q]) cand(A + D * c1);
+                        }
+                    }
+                }
+            }
+        };
+        if (mm > 1) clip(pts0, pts1);
+        if (mm > 2) {
+            clip(pts0, pts2);
+            clip(pts3, pts2);
+            clip(pts4, pts1);
+        }
+        const float a = pts1.x, bb = pts2.x, c = pts1.y, d = pts2.y;
+        const float c1 = a * d - bb * c;
+        if (mm > 2) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float llx = (i / 2) ? lx : -lx, lly = (i % 2) ? ly : -ly;
+                const float x = llx - pts0.x, y = lly - pts0.y;
+                const float u = (x * d - y * bb) / c1, v = (y * a - x * c) / c1;
+                if (0.f < u && u < 1.f && 0.f < v && v < 1.f) cand(mk3(llx, lly, pts0.z + u * pts1.z + v * pts2.z));
+            }
+        }
+        const int nv = 1 << (mm - 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (i < nv) {
+                const f3 t = i == 0 ? pts0 : (i == 1 ? pts3 : (i == 2 ? pts4 : pts5));
+                if (!(i && (t.x <= -lx || t.x >= lx || t.y <= -ly || t.y >= ly))) cand(t);
+            }
+        }
+        return;
+    }
+    // edge-edge contact (box_contact.py:458-785)
+    const int ce = code - 12;
+    const int q1 = ce / 3, q2 = ce % 3;
+    int ax1 = q2 == 1 ? 0 : 1, ax2 = q2 == 0 ? 2 : (q2 == 1 ? 2 : 0);
+    int pax1 = q1 == 1 ? 0 : 1, pax2 = q1 == 0 ? 2 : (q1 == 1 ? 2 : 0);
+    if (mel(rotabs, q1, ax1) < mel(rotabs, q1, ax2)) {
+        ax1 = ax2;
+        ax2 = 3 - q2 - ax1;
+    }
+    if (mel(rottabs, q2, pax1) < mel(rottabs, q2, pax2)) {
+        pax1 = pax2;
+        pax2 = 3 - q1 - pax1;
+    }
+    const int clface = (cle1 & (1 << pax2)) ? pax2 : pax2 + 3;
+    box_face_frame(clface, rotmore, i0, i1, i2, f0, f1, f2);
+    f3 p = rotaxis3(pos21, i0, i1, i2, f0, f1, f2);
+    const f3 rnorm = rotaxis3(clnorm, i0, i1, i2, f0, f1, f2);
+    const m33 r = rotmatx3(rot, i0, i1, i2, f0, f1, f2);
+    const f3 t1s = mv33(tr33(rotmore), size1);
+    const f3 sz = mk3(fabsf(t1s.x), fabsf(t1s.y), fabsf(t1s.z));
+    const m33 rt = tr33(r);
+    const float lx = sz.x, ly = sz.y, hz = sz.z;
+    p.z = p.z - hz;
+    const float e1 = (cle2 & (1 << ax1)) ? 1.f : -1.f, e2 = (cle2 & (1 << ax2)) ? 1.f : -1.f;
+    const f3 r_ax1 = mrow3(rt, ax1), r_ax2 = mrow3(rt, ax2), r_q2 = mrow3(rt, q2);
+    const float s_ax1 = comp(size2, ax1), s_ax2 = comp(size2, ax2), s_q2 = comp(size2, q2);
+    f3 u0 = p;
+    u0 = u0 + r_ax1 * s_ax1 * e1;
+    u0 = u0 + r_ax2 * s_ax2 * e2;
+    f3 u1 = u0;
+    u0 = u0 + r_q2 * s_q2;
+    u1 = u1 - r_q2 * s_q2;
+    f3 u2 = p;
+    u2 = u2 + r_ax1 * s_ax1 * (-e1);
+    u2 = u2 + r_ax2 * s_ax2 * e2;
+    f3 u3 = u2;
+    u2 = u2 + r_q2 * s_q2;
+    u3 = u3 - r_q2 * s_q2;
+    const f3 axi0 = u0, axi1 = u1 - u0, axi2 = u2 - u0;
+    if (fabsf(rnorm.z) < EPS) return;
+    const float innorm = (1.f / rnorm.z) * (in_ ? -1.f : 1.f);
+    // pu: the points as they were; q: projected along the normal onto the face plane
+    const f3 pu0 = u0, pu1 = u1, pu2 = u2, pu3 = u3;
+    const f3 q0 = u0 + (-u0.z / rnorm.z) * rnorm, q1p = u1 + (-u1.z / rnorm.z) * rnorm;
+    const f3 q2p = u2 + (-u2.z / rnorm.z) * rnorm, q3p = u3 + (-u3.z / rnorm.z) * rnorm;
+    const f3 pts0 = q0, pts1 = q1p - q0, pts2 = q2p - q0;
+    const m33 rr = mm33(mat1, tr33(rotmore));
+    const f3 nr = mv33(rr, rnorm);
+    emit.base = pos1;
+    emit.nrm = -((in_ ? -1.f : 1.f) * nr);
+    emit.R = rr;
+    emit.hz = hz;
+    int n = 0;
+    auto eclip = [&](f3 A, f3 D, f3 UA, f3 UD) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const float a = A[q], bb = D[q], c = A[1 - q], d = D[1 - q];
+            if (fabsf(bb) > EPS) {
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    const float l = sz[q] * (float)(2 * jj - 1);
+                    const float c1 = (l - a) / bb;
+                    if (0.f <= c1 && c1 <= 1.f) {
+                        const float c2 = c + d * c1;
+                        if (fabsf(c2) <= sz[1 - q] && (UA.z + UD.z * c1) * innorm <= margin) {
+                            f3 pt = UA * 0.5f + c1 * 0.5f * UD;
+                            if (q == 0) pt.x = pt.x + 0.5f * l, pt.y = pt.y + 0.5f * c2;
+                            else pt.y = pt.y + 0.5f * l, pt.x = pt.x + 0.5f * c2;
+                            n++;
+                            emit(pt, pt.z * innorm * 2.f);
+                        }
+                    }
+                }
+            }
+        }
+    };
+    eclip(pts0, pts1, axi0, axi1);
+    eclip(pts0, pts2, axi0, axi2);
+    eclip(pts0 + pts1, pts2, axi0 + axi1, axi2);
+    eclip(pts0 + pts2, pts1, axi0 + axi2, axi1);
+    const int nl = n;
+    {
+        const float a = pts1.x, bb = pts2.x, c = pts1.y, d = pts2.y;
+        const float c1 = a * d - bb * c;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float llx = (i / 2) ? lx : -lx, lly = (i % 2) ? ly : -ly;
+            const float x = llx - pts0.x, y = lly - pts0.y;
+            float u = (x * d - y * bb) / c1, v = (y * a - x * c) / c1;
+            bool skip;
+            if (nl == 0) skip = (u < 0.f || u > 1.f) && (v < 0.f || v > 1.f);
+            else skip = u < 0.f || u > 1.f || v < 0.f || v > 1.f;
+            if (skip) continue;
+            u = fminf(fmaxf(u, 0.f), 1.f);
+            v = fminf(fmaxf(v, 0.f), 1.f);
+            const f3 t1 = pu0 * (1.f - u - v) + pu1 * u + pu2 * v;
+            f3 pt = mk3(llx, lly, 0.f);
+            const f3 t2 = pt - t1;
+            const float cc = dot3(t2, t2);
+            if (!(t1.z > 0.f && cc > margin2)) {
+                pt = pt + t1;
+                pt = pt * 0.5f;
+                n++;
+                emit(pt, sqrtf(cc) * (t1.z < 0.f ? -1.f : 1.f));
+            }
+        }
+    }
+    const int nf = n;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const f3 qi = i == 0 ? q0 : (i == 1 ? q1p : (i == 2 ? q2p : q3p));
+        const f3 pui = i == 0 ? pu0 : (i == 1 ? pu1 : (i == 2 ? pu2 : pu3));
+        const float x = qi.x, y = qi.y;
+        bool skip;
+        if (nl == 0) skip = nf != 0 && (x < -lx || x > lx) && (y < -ly || y > ly);
+        else skip = x < -lx || x > lx || y < -ly || y > ly;
+        if (skip) continue;
+        float cc = 0.f;
+        if (x < -sz.x) cc = cc + (x + sz.x) * (x + sz.x);
+        else if (x > sz.x) cc = cc + (x - sz.x) * (x - sz.x);
+        if (y < -sz.y) cc = cc + (y + sz.y) * (y + sz.y);
+        else if (y > sz.y) cc = cc + (y - sz.y) * (y - sz.y);
+        cc = cc + (pui.z * innorm) * (pui.z * innorm);
+        if (pui.z > 0.f && cc > margin2) continue;
+        f3 t1 = mk3(x * 0.5f, y * 0.5f, 0.f);
+        if (x < -sz.x) t1.x = -sz.x * 0.5f;
+        else if (x > sz.x) t1.x = sz.x * 0.5f;
+        if (y < -sz.y) t1.y = -sz.y * 0.5f;
+        else if (y > sz.y) t1.y = sz.y * 0.5f;
+        t1 = t1 + pui * 0.5f;
+        n++;
+        emit(t1, sqrtf(cc) * (pui.z < 0.f ? -1.f : 1.f));
+    }
+}
+
 // narrowphase.py:1988-2060 + :2135-2189
 __device__ inline bool convex_pair(const DModel& m, int ta, int tb, int ga, int gb) {
     return m.geom_is_convex[ga] && m.geom_is_convex[gb] && tb != GT_TERRAIN &&
@@ -825,6 +1203,7 @@ __device__ inline void dev_narrowphase(const DModel& m, const S& st, int b) {
             gb = t;
         }
         if (m.geom_type[ga] == GT_PLANE && m.geom_type[gb] == GT_BOX) plane_box(m, st, b, ga, gb, sink);
+        if (m.box_box_detection && m.geom_type[ga] == GT_BOX && m.geom_type[gb] == GT_BOX) box_box(m, st, b, ga, gb, sink);
     }
 }
 

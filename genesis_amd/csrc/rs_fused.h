@@ -197,6 +197,7 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
                 pair = m.collision_pair_idx[min(ga, gb) * m.G + max(ga, gb)];
                 if (pass == 0) conv = convex_pair(m, ta, tb, ga, gb) && convex_setup(m, ga, gb, f);
                 else if (ta == GT_PLANE && tb == GT_BOX) plane_box(m, loc, 0, ga, gb, sink);
+                else if (m.box_box_detection && ta == GT_BOX && tb == GT_BOX) box_box(m, loc, 0, ga, gb, sink);
             }
             if (pass == 0) {
                 // round 0: detection 0 of this lane's pair; round 1: the perturbed detections, one (pair, det) item

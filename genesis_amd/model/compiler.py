@@ -371,9 +371,8 @@ def compile_model(entities: list[EntitySpec], options: SolverOptions | None = No
 
     # ------------------------------------------------------------------ capacities
     n_box = int((gtype == GEOM_BOX).sum())
-    model.n_contacts_per_pair = 5
-    if opt.box_box_detection and n_box > 1:
-        raise NotImplementedError("box_box_detection=True: box-box separating-axis detection is not built")
+    # collider.py:145-150: box-box clipping may add up to 16 points per pair
+    model.n_contacts_per_pair = 16 if (opt.box_box_detection and n_box > 1) else 5
     model.max_collision_pairs = min(opt.max_collision_pairs, model.n_pairs)
     model.max_contact_pairs = model.max_collision_pairs * model.n_contacts_per_pair
     model.max_collision_pairs_broad = model.max_collision_pairs * opt.multiplier_collision_broad_phase

@@ -83,8 +83,6 @@ class RigidOptions(_Options):
             # the decomposed solve with a parallel grid line search (constraint/solver_breakdown.py:812-876) is not
             # built; None lets the solver choose, and the choice here is always the monolithic solve
             "prefer_parallel_linesearch": (bool(self.prefer_parallel_linesearch), False),
-            # box-box separating-axis detection (collider/box_contact.py:100) is not built
-            "box_box_detection": (bool(self.box_box_detection), False),
             # SAP is the only traversal built; None lets the solver choose (constants.py:65-98)
             "broadphase_traversal": (int(self.broadphase_traversal or 0), 0),
         }

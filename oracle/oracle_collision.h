@@ -696,6 +696,385 @@ template <class T> void plane_box_contact(const Model<T>& m, Env<T>& s, int ga, 
     }
 }
 
+// contact.py:16-31 rotaxis / rotmatx
+template <class T> inline V3<T> rotaxis(V3<T> v, int i0, int i1, int i2, T f0, T f1, T f2) {
+    return v3(v[i0] * f0, v[i1] * f1, v[i2] * f2);
+}
+template <class T> inline M3<T> rotmatx(const M3<T>& a, int i0, int i1, int i2, T f0, T f1, T f2) {
+    M3<T> r;
+    for (int c = 0; c < 3; c++) {
+        r.m[0][c] = a.m[i0][c] * f0;
+        r.m[1][c] = a.m[i1][c] * f1;
+        r.m[2][c] = a.m[i2][c] * f2;
+    }
+    return r;
+}
+template <class T> inline M3<T> mtrans(const M3<T>& a) {
+    M3<T> r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.m[i][j] = a.m[j][i];
+    return r;
+}
+template <class T> inline M3<T> mmul(const M3<T>& a, const M3<T>& b) {
+    M3<T> r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+    return r;
+}
+template <class T> inline V3<T> mvec(const M3<T>& a, V3<T> v) {
+    return v3(a.m[0][0] * v[0] + a.m[0][1] * v[1] + a.m[0][2] * v[2], a.m[1][0] * v[0] + a.m[1][1] * v[1] + a.m[1][2] * v[2],
+              a.m[2][0] * v[0] + a.m[2][1] * v[1] + a.m[2][2] * v[2]);
+}
+template <class T> inline V3<T> mrow(const M3<T>& a, int i) { return v3(a.m[i][0], a.m[i][1], a.m[i][2]); }
+
+// box_contact.py:251-303 / :488-520: the frame that puts the clipping face on +z (face index q in 0..5)
+template <class T> inline void box_face_frame(int q, M3<T>& rotmore, int& i0, int& i1, int& i2, T& f0, T& f1, T& f2) {
+    rotmore = M3<T>{{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}}};
+    i0 = 0, i1 = 1, i2 = 2;
+    f0 = f1 = f2 = T(1);
+    if (q == 0) rotmore.m[0][2] = T(-1), rotmore.m[1][1] = T(1), rotmore.m[2][0] = T(1), i0 = 2, f0 = T(-1), i2 = 0;
+    else if (q == 1) rotmore.m[0][0] = T(1), rotmore.m[1][2] = T(-1), rotmore.m[2][1] = T(1), i1 = 2, f1 = T(-1), i2 = 1;
+    else if (q == 2) rotmore.m[0][0] = T(1), rotmore.m[1][1] = T(1), rotmore.m[2][2] = T(1);
+    else if (q == 3) rotmore.m[0][2] = T(1), rotmore.m[1][1] = T(1), rotmore.m[2][0] = T(-1), i0 = 2, i2 = 0, f2 = T(-1);
+    else if (q == 4) rotmore.m[0][0] = T(1), rotmore.m[1][2] = T(1), rotmore.m[2][1] = T(-1), i1 = 2, i2 = 1, f2 = T(-1);
+    else if (q == 5) rotmore.m[0][0] = T(-1), rotmore.m[1][1] = T(1), rotmore.m[2][2] = T(-1), f0 = T(-1), f2 = T(-1);
+}
+
+// box_contact.py:100-785 func_box_box_contact (MuJoCo's engine_collision_box.c separating-axis test, then face or edge
+// clipping). The reference keeps up to n_contacts_per_pair candidate points and writes past that array when a
+// configuration yields more (it does not check); this restatement keeps every candidate (at most 24).
+template <class T> void box_box_contact(const Model<T>& m, Env<T>& s, int ga, int gb) {
+    using E = Env<T>;
+    const T EPS = m.EPS;
+    const T margin = T(0), margin2 = margin * margin;
+    int pair = m.collision_pair_idx[(ga < gb ? ga : gb) * m.G + (ga < gb ? gb : ga)];
+    int n = 0, code = -1, cle1 = 0, cle2 = 0;
+    bool is_return = false, in_ = false;
+    M3<T> rotmore{{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}}};
+    const V3<T> pos1 = E::g3(s.geoms_pos, ga), pos2 = E::g3(s.geoms_pos, gb);
+    const Q4<T> q1_ = E::g4(s.geoms_quat, ga), q2_ = E::g4(s.geoms_quat, gb);
+    const V3<T> size1 = v3(m.geom_data[7 * ga] / T(2), m.geom_data[7 * ga + 1] / T(2), m.geom_data[7 * ga + 2] / T(2));
+    const V3<T> size2 = v3(m.geom_data[7 * gb] / T(2), m.geom_data[7 * gb + 1] / T(2), m.geom_data[7 * gb + 2] / T(2));
+    const M3<T> mat1 = quat_to_R(q1_, EPS), mat2 = quat_to_R(q2_, EPS);
+    const V3<T> pos21 = mvec(mtrans(mat1), pos2 - pos1);
+    const V3<T> pos12 = mvec(mtrans(mat2), pos1 - pos2);
+    const M3<T> rot = mmul(mtrans(mat1), mat2), rott = mtrans(rot);
+    M3<T> rotabs, rottabs;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) rotabs.m[i][j] = std::fabs(rot.m[i][j]), rottabs.m[i][j] = std::fabs(rott.m[i][j]);
+    const V3<T> plen2 = mvec(rotabs, size2);
+    const V3<T> plen1 = mvec(mtrans(rotabs), size1);
+    T penetration = margin;
+    for (int i = 0; i < 3; i++) penetration = penetration + size1[i] * T(3) + size2[i] * T(3);
+    // face axes of either box (box_contact.py:162-175)
+    for (int i = 0; i < 3; i++) {
+        T c1 = -std::fabs(pos21[i]) + size1[i] + plen2[i];
+        T c2 = -std::fabs(pos12[i]) + size2[i] + plen1[i];
+        if (c1 < -margin || c2 < -margin) is_return = true;
+        if (c1 < penetration) {
+            penetration = c1;
+            code = i + 3 * (pos21[i] < T(0) ? 1 : 0) + 0;
+        }
+        if (c2 < penetration) {
+            penetration = c2;
+            code = i + 3 * (pos12[i] < T(0) ? 1 : 0) + 6;
+        }
+    }
+    // edge-edge axes (box_contact.py:176-238)
+    V3<T> clnorm = v3(T(0), T(0), T(0));
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            const T rj0 = rott.m[j][0], rj1 = rott.m[j][1], rj2 = rott.m[j][2];
+            V3<T> tmp2 = i == 0 ? v3(T(0), -rj2, rj1) : (i == 1 ? v3(rj2, T(0), -rj0) : v3(-rj1, rj0, T(0)));
+            const T c1 = norm(tmp2);
+            tmp2 = tmp2 / c1;
+            if (c1 >= EPS) {
+                const T c2 = dot(pos21, tmp2);
+                T c3 = T(0);
+                for (int k = 0; k < 3; k++)
+                    if (k != i) c3 = c3 + size1[k] * std::fabs(tmp2[k]);
+                for (int k = 0; k < 3; k++)
+                    if (k != j) c3 = c3 + size2[k] * rotabs.m[i][3 - k - j] / c1;
+                c3 = c3 - std::fabs(c2);
+                if (c3 < -margin) is_return = true;
+                if (c3 < penetration * (T(1) - T(1e-12))) {
+                    penetration = c3;
+                    cle1 = 0;
+                    for (int k = 0; k < 3; k++)
+                        if (k != i && ((tmp2[k] > T(0)) != (c2 < T(0)))) cle1 += 1 << k;
+                    cle2 = 0;
+                    for (int k = 0; k < 3; k++)
+                        if (k != j && (((rot.m[i][3 - k - j] > T(0)) != (c2 < T(0))) != ((k - j + 3) % 3 == 1)))
+                            cle2 += 1 << k;
+                    code = 12 + i * 3 + j;
+                    clnorm = tmp2;
+                    in_ = c2 < T(0);
+                }
+            }
+        }
+    }
+    if (code == -1) is_return = true;
+    if (is_return) return;
+    V3<T> points[24];
+    T depth[24];
+    if (code < 12) {
+        // face contact (box_contact.py:241-457)
+        const int q1 = code % 6, q2 = code / 6;
+        int i0, i1, i2;
+        T f0, f1, f2;
+        box_face_frame(q1, rotmore, i0, i1, i2, f0, f1, f2);
+        M3<T> r;
+        V3<T> p, tmp1, sz;
+        if (q2) {
+            r = mmul(rotmore, mtrans(rot));
+            p = rotaxis(pos12, i0, i1, i2, f0, f1, f2);
+            tmp1 = rotaxis(size2, i0, i1, i2, f0, f1, f2);
+            sz = size1;
+        } else {
+            r = rotmatx(rot, i0, i1, i2, f0, f1, f2);
+            p = rotaxis(pos21, i0, i1, i2, f0, f1, f2);
+            tmp1 = rotaxis(size1, i0, i1, i2, f0, f1, f2);
+            sz = size2;
+        }
+        const M3<T> rt = mtrans(r);
+        const V3<T> ss = v3(std::fabs(tmp1[0]), std::fabs(tmp1[1]), std::fabs(tmp1[2]));
+        const T lx = ss[0], ly = ss[1], hz = ss[2];
+        p[2] = p[2] - hz;
+        V3<T> lp = p;
+        int clcorner = 0;
+        for (int i = 0; i < 3; i++)
+            if (r.m[2][i] < T(0)) clcorner += 1 << i;
+        for (int i = 0; i < 3; i++) lp = lp + mrow(rt, i) * sz[i] * T((clcorner & (1 << i)) ? 1 : -1);
+        V3<T> pts[6];
+        for (int i = 0; i < 6; i++) pts[i] = v3(T(0), T(0), T(0));
+        int mm = 0, k = 0;
+        pts[mm++] = lp;
+        for (int i = 0; i < 3; i++)
+            if (std::fabs(r.m[2][i]) < T(0.5)) pts[mm++] = mrow(rt, i) * sz[i] * T((clcorner & (1 << i)) ? -2 : 2);
+        pts[3] = pts[0] + pts[1];
+        pts[4] = pts[0] + pts[2];
+        pts[5] = pts[3] + pts[2];
+        V3<T> la[4], lb[4];
+        if (mm > 1) la[k] = pts[0], lb[k] = pts[1], k++;
+        if (mm > 2) {
+            la[k] = pts[0], lb[k] = pts[2], k++;
+            la[k] = pts[3], lb[k] = pts[2], k++;
+            la[k] = pts[4], lb[k] = pts[1], k++;
+        }
+        for (int i = 0; i < k; i++)
+            for (int q = 0; q < 2; q++) {
+                const T a = la[i][q], b = lb[i][q], c = la[i][1 - q], d = lb[i][1 - q];
+                if (std::fabs(b) > EPS)
+                    for (int jj = 0; jj < 2; jj++) {
+                        const T l = ss[q] * T(2 * jj - 1);
+                        const T c1 = (l - a) / b;
+                        if (T(0) <= c1 && c1 <= T(1)) {
+                            const T c2 = c + d * c1;
+                            if (std::fabs(c2) <= ss[1 - q]) points[n++] = la[i] + lb[i] * c1;
+                        }
+                    }
+            }
+        const T a = pts[1][0], b = pts[2][0], c = pts[1][1], d = pts[2][1];
+        const T c1 = a * d - b * c;
+        if (mm > 2)
+            for (int i = 0; i < 4; i++) {
+                const T llx = (i / 2) ? lx : -lx, lly = (i % 2) ? ly : -ly;
+                const T x = llx - pts[0][0], y = lly - pts[0][1];
+                const T u = (x * d - y * b) / c1, v = (y * a - x * c) / c1;
+                if (T(0) < u && u < T(1) && T(0) < v && v < T(1))
+                    points[n++] = v3(llx, lly, pts[0][2] + u * pts[1][2] + v * pts[2][2]);
+            }
+        for (int i = 0; i < (1 << (mm - 1)); i++) {
+            const V3<T> t = pts[i == 0 ? 0 : i + 2];
+            if (!(i && (t[0] <= -lx || t[0] >= lx || t[1] <= -ly || t[1] >= ly))) points[n++] = t;
+        }
+        mm = n;
+        n = 0;
+        for (int i = 0; i < mm; i++)
+            if (points[i][2] <= margin) {
+                points[n] = points[i];
+                depth[n] = points[n][2];
+                points[n][2] = points[n][2] * T(0.5);
+                n++;
+            }
+        const M3<T> rr = mmul(q2 ? mat2 : mat1, mtrans(rotmore));
+        const V3<T> pp = q2 ? pos2 : pos1;
+        const T sg = q2 ? T(-1) : T(1);
+        const V3<T> normal0 = v3(sg * rr.m[0][2], sg * rr.m[1][2], sg * rr.m[2][2]);
+        int n_added = 0;
+        const int n_start = s.n_contacts;
+        for (int i = 0; i < n; i++) {
+            if (n_added < m.h.n_contacts_per_pair) {
+                // the reference takes the halved height as the distance here (box_contact.py:434)
+                const T dist = points[i][2];
+                points[i][2] = points[i][2] + hz;
+                const V3<T> cpos = pp + mvec(rr, points[i]);
+                bool valid = true;
+                for (int j = 0; j < n_added && n_start + j < m.KMAX; j++) {
+                    const V3<T> q = E::g3(s.contact_pos, n_start + j);
+                    if (std::fabs(cpos[0] - q[0]) < EPS && std::fabs(cpos[1] - q[1]) < EPS && std::fabs(cpos[2] - q[2]) < EPS)
+                        valid = false;
+                }
+                if (valid) {
+                    add_contact(m, s, ga, gb, -normal0, cpos, -dist, pair);
+                    n_added++;
+                }
+            }
+        }
+        return;
+    }
+    // edge-edge contact (box_contact.py:458-785)
+    code -= 12;
+    const int q1 = code / 3, q2 = code % 3;
+    int ax1 = 0, ax2 = 0, pax1 = 0, pax2 = 0;
+    if (q2 == 0) ax1 = 1, ax2 = 2;
+    else if (q2 == 1) ax1 = 0, ax2 = 2;
+    else ax1 = 1, ax2 = 0;
+    if (q1 == 0) pax1 = 1, pax2 = 2;
+    else if (q1 == 1) pax1 = 0, pax2 = 2;
+    else pax1 = 1, pax2 = 0;
+    if (rotabs.m[q1][ax1] < rotabs.m[q1][ax2]) {
+        ax1 = ax2;
+        ax2 = 3 - q2 - ax1;
+    }
+    if (rottabs.m[q2][pax1] < rottabs.m[q2][pax2]) {
+        pax1 = pax2;
+        pax2 = 3 - q1 - pax1;
+    }
+    const int clface = (cle1 & (1 << pax2)) ? pax2 : pax2 + 3;
+    int i0, i1, i2;
+    T f0, f1, f2;
+    box_face_frame(clface, rotmore, i0, i1, i2, f0, f1, f2);
+    V3<T> p = rotaxis(pos21, i0, i1, i2, f0, f1, f2);
+    const V3<T> rnorm = rotaxis(clnorm, i0, i1, i2, f0, f1, f2);
+    const M3<T> r = rotmatx(rot, i0, i1, i2, f0, f1, f2);
+    const V3<T> tmp1s = mvec(mtrans(rotmore), size1);
+    const V3<T> sz = v3(std::fabs(tmp1s[0]), std::fabs(tmp1s[1]), std::fabs(tmp1s[2]));
+    const M3<T> rt = mtrans(r);
+    const T lx = sz[0], ly = sz[1], hz = sz[2];
+    p[2] = p[2] - hz;
+    const T e1 = (cle2 & (1 << ax1)) ? T(1) : T(-1), e2 = (cle2 & (1 << ax2)) ? T(1) : T(-1);
+    V3<T> pq[4];
+    pq[0] = p;
+    pq[0] = pq[0] + mrow(rt, ax1) * size2[ax1] * e1;
+    pq[0] = pq[0] + mrow(rt, ax2) * size2[ax2] * e2;
+    pq[1] = pq[0];
+    pq[0] = pq[0] + mrow(rt, q2) * size2[q2];
+    pq[1] = pq[1] - mrow(rt, q2) * size2[q2];
+    pq[2] = p;
+    pq[2] = pq[2] + mrow(rt, ax1) * size2[ax1] * (-e1);
+    pq[2] = pq[2] + mrow(rt, ax2) * size2[ax2] * e2;
+    pq[3] = pq[2];
+    pq[2] = pq[2] + mrow(rt, q2) * size2[q2];
+    pq[3] = pq[3] - mrow(rt, q2) * size2[q2];
+    const V3<T> axi0 = pq[0], axi1 = pq[1] - pq[0], axi2 = pq[2] - pq[0];
+    if (std::fabs(rnorm[2]) < EPS) return;
+    const T innorm = (T(1) / rnorm[2]) * (in_ ? T(-1) : T(1));
+    V3<T> pu[4];
+    T ppts2[4][2];
+    for (int i = 0; i < 4; i++) {
+        const T c1 = -pq[i][2] / rnorm[2];
+        pu[i] = pq[i];
+        pq[i] = pq[i] + c1 * rnorm;
+        ppts2[i][0] = pq[i][0];
+        ppts2[i][1] = pq[i][1];
+    }
+    const V3<T> pts0 = pq[0], pts1 = pq[1] - pq[0], pts2 = pq[2] - pq[0];
+    V3<T> la[4] = {pts0, pts0, pts0 + pts1, pts0 + pts2}, lb[4] = {pts1, pts2, pts2, pts1};
+    V3<T> ua[4] = {axi0, axi0, axi0 + axi1, axi0 + axi2}, ub[4] = {axi1, axi2, axi2, axi1};
+    for (int i = 0; i < 4; i++)
+        for (int q = 0; q < 2; q++) {
+            const T a = la[i][q], b = lb[i][q], c = la[i][1 - q], d = lb[i][1 - q];
+            if (std::fabs(b) > EPS)
+                for (int jj = 0; jj < 2; jj++) {
+                    const T l = sz[q] * T(2 * jj - 1);
+                    const T c1 = (l - a) / b;
+                    if (T(0) <= c1 && c1 <= T(1)) {
+                        const T c2 = c + d * c1;
+                        if (std::fabs(c2) <= sz[1 - q] && (ua[i][2] + ub[i][2] * c1) * innorm <= margin) {
+                            points[n] = ua[i] * T(0.5) + c1 * T(0.5) * ub[i];
+                            points[n][q] = points[n][q] + T(0.5) * l;
+                            points[n][1 - q] = points[n][1 - q] + T(0.5) * c2;
+                            depth[n] = points[n][2] * innorm * T(2);
+                            n++;
+                        }
+                    }
+                }
+        }
+    const int nl = n;
+    const T a = pts1[0], b = pts2[0], c = pts1[1], d = pts2[1];
+    const T c1 = a * d - b * c;
+    for (int i = 0; i < 4; i++) {
+        const T llx = (i / 2) ? lx : -lx, lly = (i % 2) ? ly : -ly;
+        const T x = llx - pts0[0], y = lly - pts0[1];
+        T u = (x * d - y * b) / c1, v = (y * a - x * c) / c1;
+        if (nl == 0) {
+            if ((u < T(0) || u > T(1)) && (v < T(0) || v > T(1))) continue;
+        } else if (u < T(0) || u > T(1) || v < T(0) || v > T(1)) {
+            continue;
+        }
+        u = std::fmin(std::fmax(u, T(0)), T(1));
+        v = std::fmin(std::fmax(v, T(0)), T(1));
+        const V3<T> t1 = pu[0] * (T(1) - u - v) + pu[1] * u + pu[2] * v;
+        points[n] = v3(llx, lly, T(0));
+        const V3<T> t2 = points[n] - t1;
+        const T cc = dot(t2, t2);
+        if (!(t1[2] > T(0) && cc > margin2)) {
+            points[n] = points[n] + t1;
+            points[n] = points[n] * T(0.5);
+            depth[n] = std::sqrt(cc) * (t1[2] < T(0) ? T(-1) : T(1));
+            n++;
+        }
+    }
+    const int nf = n;
+    for (int i = 0; i < 4; i++) {
+        const T x = ppts2[i][0], y = ppts2[i][1];
+        if (nl == 0) {
+            if (nf != 0 && (x < -lx || x > lx) && (y < -ly || y > ly)) continue;
+        } else if (x < -lx || x > lx || y < -ly || y > ly) {
+            continue;
+        }
+        T cc = T(0);
+        for (int j = 0; j < 2; j++) {
+            if (ppts2[i][j] < -sz[j]) cc = cc + (ppts2[i][j] + sz[j]) * (ppts2[i][j] + sz[j]);
+            else if (ppts2[i][j] > sz[j]) cc = cc + (ppts2[i][j] - sz[j]) * (ppts2[i][j] - sz[j]);
+        }
+        cc = cc + (pu[i][2] * innorm) * (pu[i][2] * innorm);
+        if (pu[i][2] > T(0) && cc > margin2) continue;
+        V3<T> t1 = v3(ppts2[i][0] * T(0.5), ppts2[i][1] * T(0.5), T(0));
+        for (int j = 0; j < 2; j++) {
+            if (ppts2[i][j] < -sz[j]) t1[j] = -sz[j] * T(0.5);
+            else if (ppts2[i][j] > sz[j]) t1[j] = sz[j] * T(0.5);
+        }
+        t1 = t1 + pu[i] * T(0.5);
+        points[n] = t1;
+        depth[n] = std::sqrt(cc) * (pu[i][2] < T(0) ? T(-1) : T(1));
+        n++;
+    }
+    const M3<T> rr = mmul(mat1, mtrans(rotmore));
+    const V3<T> nr = mvec(rr, rnorm);
+    const V3<T> normal0 = (in_ ? T(-1) : T(1)) * nr;
+    int n_added = 0;
+    const int n_start = s.n_contacts;
+    for (int i = 0; i < n; i++) {
+        if (n_added < m.h.n_contacts_per_pair) {
+            const T dist = depth[i];
+            points[i][2] = points[i][2] + hz;
+            const V3<T> cpos = pos1 + mvec(rr, points[i]);
+            bool valid = true;
+            for (int j = 0; j < n_added && n_start + j < m.KMAX; j++) {
+                const V3<T> q = E::g3(s.contact_pos, n_start + j);
+                if (std::fabs(cpos[0] - q[0]) < EPS && std::fabs(cpos[1] - q[1]) < EPS && std::fabs(cpos[2] - q[2]) < EPS)
+                    valid = false;
+            }
+            if (valid) {
+                add_contact(m, s, ga, gb, -normal0, cpos, -dist, pair);
+                n_added++;
+            }
+        }
+    }
+}
+
 // narrowphase.py:1988-2060 func_narrow_phase_convex_vs_convex + :2135-2189 specializations
 template <class T> void narrowphase(const Model<T>& m, Env<T>& s) {
     for (int i = 0; i < s.n_broad; i++) {
@@ -710,6 +1089,7 @@ template <class T> void narrowphase(const Model<T>& m, Env<T>& s) {
         int ga = s.broad_pairs[2 * i], gb = s.broad_pairs[2 * i + 1];
         if (m.geom_type[ga] > m.geom_type[gb]) std::swap(ga, gb);
         if (m.geom_type[ga] == GT_PLANE && m.geom_type[gb] == GT_BOX) plane_box_contact(m, s, ga, gb);
+        if (m.h.box_box_detection && m.geom_type[ga] == GT_BOX && m.geom_type[gb] == GT_BOX) box_box_contact(m, s, ga, gb);
     }
 }
 

@@ -81,10 +81,10 @@ def test_unbuilt_options_are_rejected():
 
     from genesis_amd.options import RigidOptions
 
-    for kw in (dict(box_box_detection=True), dict(prefer_parallel_linesearch=True), dict(broadphase_traversal=1),
+    for kw in (dict(prefer_parallel_linesearch=True), dict(broadphase_traversal=1),
                dict(use_gjk_collision=True)):
         with pytest.raises(NotImplementedError):
             RigidOptions(**kw).validate()
     for kw in (dict(prefer_parallel_linesearch=None), dict(prefer_parallel_linesearch=False),
-               dict(broadphase_traversal=None), dict(broadphase_traversal=0)):
+               dict(broadphase_traversal=None), dict(broadphase_traversal=0), dict(box_box_detection=True)):
         RigidOptions(**kw).validate()

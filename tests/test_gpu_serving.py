@@ -6,6 +6,7 @@ deterministic, so the check is bit-exact.
 
 from __future__ import annotations
 
+import numpy as np
 import pytest
 import torch
 
@@ -47,3 +48,36 @@ def test_budget_admission_queues_and_completes():
     assert stats["requests"] == len(reqs)
     assert server.used == 0
     assert stats["p99_step_ms"] >= stats["p50_step_ms"] > 0
+
+
+def test_serving_mix_at_its_stated_size():
+    """The serving-mix workload as stated: 24 seeded Poisson requests at 100/s, scenes uniform over franka / box pile /
+    quadruped, n_envs log-uniform in [64, 4096], horizons uniform in [50, 500]. Every request ends bit-identical to its
+    run alone; and each request's alone run follows the FP32 restatement over its first 5 substeps (per-environment
+    max |q_device - q_restatement|, 99% of the environments within 1e-3, every environment within 5e-2: contact-rich
+    states amplify FP32 rounding differences)."""
+    from tests.oracle_lib import Oracle
+    from tests.parity import get_dev, slab_to_oracle
+
+    reqs = poisson_requests(24, rate_hz=100.0, seed=4, min_envs=64, max_envs=4096, min_horizon=50, max_horizon=500)
+    server = RolloutServer(hbm_budget_bytes=64 << 30, max_active=32)
+    print(f"[serving] stated mix: {sum(r.n_envs * r.horizon for r in reqs)} env-steps", flush=True)
+    stats = server.run(reqs, keep_final=True)
+    assert stats["requests"] == 24 and server.used == 0
+    print(f"[serving] {stats['env_steps_per_s']:.3e} env-steps/s, mean active {stats['mean_active']:.1f}, "
+          f"tick p50 {stats['p50_step_ms']:.2f} ms p99 {stats['p99_step_ms']:.2f} ms", flush=True)
+    worst = []
+    for r in reqs:
+        q, v = _alone(server, r)
+        assert torch.equal(q, r.final_qpos) and torch.equal(v, r.final_vel), f"request {r.rid} differs from its run alone"
+        slab = server.new_slab(r)
+        oracle = Oracle(slab.dmodel.model, r.n_envs, precision=32)
+        slab_to_oracle(slab, oracle)
+        slab.step(5, rt.RS_MODE_FUSED)
+        torch.cuda.synchronize()
+        oracle.step(5, 16)
+        d = np.abs(get_dev(slab, "qpos") - oracle.get("qpos")).max(axis=1)
+        worst.append(float(d.max()))
+        print(f"[serving] request {r.rid} ({r.scene}, {r.n_envs} envs, {r.horizon} steps): alone == mixed; 5 steps vs "
+              f"restatement: p99 {np.percentile(d, 99):.2e} max {d.max():.2e}", flush=True)
+        assert np.percentile(d, 99) <= 1e-3 and d.max() <= 5e-2, r.rid
