@@ -40,9 +40,10 @@ class Workload:
             self.B, self.steps = 4096, 1000
             self.desc = "Franka Panda x{B}, plane, dt=0.01, PD targets resampled every 100 steps"
         elif name == "boxpile":
-            self.model = scenes.box_pile_model()
+            # box-box pairs through the separating-axis detection (box_box_detection=True), as the workload asks
+            self.model = scenes.box_pile_model(box_box_detection=True)
             self.B, self.steps = 1024, 500
-            self.desc = "box pile (20 free boxes) x{B}, plane, dt=0.01"
+            self.desc = "box pile (20 free boxes, box-box SAT detection) x{B}, plane, dt=0.01"
         elif name == "quadruped":
             self.model = scenes.quadruped_model()
             self.B, self.steps = 8192, 1000
