@@ -29,6 +29,9 @@ from genesis_amd import runtime as rt  # noqa: E402
 from genesis_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (counts an FMA as 2 and packed FP32)
+FLOP_FILES = {"franka": "flops_franka4096.json", "boxpile": "flops_boxpile1024.json",
+              "quadruped": "flops_quadruped8192.json"}
 BLOCK = 100  # substeps per captured graph (and per target block)
 
 
@@ -76,6 +79,23 @@ def algorithmic_bytes_per_env_step(model, mean_overlap_pairs):
     warm-start normal cache r+w (24 B) of each AABB-overlapping pair."""
     Q, D = model.n_qs, model.n_dofs
     return 8 * Q + 8 * D + 4 * D + 8 * D + 12 * D + 8 + 4 + 2 + 24 * mean_overlap_pairs
+
+
+def valu_roofline(wl, kernel_ms, B):
+    """FP32 work of the dominant kernel against the vector peak: the FLOPs per env-step of this workload's step,
+    counted once by the restatement on an operation-counting double over the bench's own run (profiles/flops_*.json,
+    written by tests/flop_count.py), times the environments one launch steps, over the launch's measured time."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", FLOP_FILES[wl.name])
+    try:
+        with open(path) as f:
+            fl = json.load(f)
+    except (OSError, ValueError):
+        return None
+    achieved = fl["flops_per_env_step"] * B / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "flops_per_env_step": fl["flops_per_env_step"],
+            "per_stage_flops_per_env_step": fl["per_stage_flops_per_env_step"],
+            "source": f"profiles/{FLOP_FILES[wl.name]} ({fl['sample']})"}
 
 
 def make_slab(wl, dm, B):
@@ -426,6 +446,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_env_step": bytes_per_env_step, "mean_overlap_pairs": nb_f,
                      "kernel": "k_fused_wave", "kernel_ms": kernel_ms},
+        "valu_roofline": valu_roofline(wl, kernel_ms, B),
         "errno": errno,
     }
     if args.workload != "franka":

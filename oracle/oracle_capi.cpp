@@ -6,6 +6,8 @@
 #include <string>
 #include <thread>
 
+#include "oracle_flops.h"  // before the restatement: its std:: overloads must be visible to the templates
+
 #include "oracle_collision.h"
 #include "oracle_constraint.h"
 #include "oracle_dynamics.h"
@@ -167,10 +169,13 @@ using namespace orc;
 template <class T> static int with_ctx(void* h, std::function<int(Context<T>&)> f) {
     return f(*reinterpret_cast<Context<T>*>(reinterpret_cast<Handle*>(h)->ctx));
 }
+// precision 64: double, 32: float, 0: the operation-counting double (oracle_flops.h)
 #define DISPATCH(h, body)                                                              \
     (reinterpret_cast<Handle*>(h)->precision == 64                                     \
          ? with_ctx<double>(h, [&](Context<double>& c) -> int { body })                \
-         : with_ctx<float>(h, [&](Context<float>& c) -> int { body }))
+         : reinterpret_cast<Handle*>(h)->precision == 0                                \
+               ? with_ctx<CF>(h, [&](Context<CF>& c) -> int { body })                  \
+               : with_ctx<float>(h, [&](Context<float>& c) -> int { body }))
 
 template <class T>
 static int pair_contact(Context<T>& c, int ga, int gb, const double* pose_a, const double* pose_b, int mpr,
@@ -204,6 +209,13 @@ void* orc_create(const rs_model_header* hdr, const void* blob, int precision, in
         c->envs.resize(n_envs);
         for (auto& e : c->envs) c->init_env(e);
         h->ctx = c;
+    } else if (precision == 0) {
+        auto* c = new Context<CF>();
+        if (!load_model(c->m, hdr, (const uint8_t*)blob)) return nullptr;
+        c->register_fields();
+        c->envs.resize(n_envs);
+        for (auto& e : c->envs) c->init_env(e);
+        h->ctx = c;
     } else {
         auto* c = new Context<float>();
         if (!load_model(c->m, hdr, (const uint8_t*)blob)) return nullptr;
@@ -232,6 +244,7 @@ int orc_set_options(void* h, const double* o) {
 void orc_destroy(void* h) {
     Handle* hh = reinterpret_cast<Handle*>(h);
     if (hh->precision == 64) delete reinterpret_cast<Context<double>*>(hh->ctx);
+    else if (hh->precision == 0) delete reinterpret_cast<Context<CF>*>(hh->ctx);
     else delete reinterpret_cast<Context<float>*>(hh->ctx);
     delete hh;
 }
@@ -316,11 +329,18 @@ int orc_step(void* h, int n_steps, int n_threads) {
 // GJK/EPA path (gjk.py:170-446). out = normal[3], position[3], penetration; returns 1 when they collide. With
 // mpr = 1 the MPR path (mpr.py:769-818) runs instead, without a warm-start direction.
 int orc_pair_contact(void* h, int ga, int gb, const double* pose_a, const double* pose_b, int mpr, double* out) {
+    if (reinterpret_cast<Handle*>(h)->precision == 0) return -1;
     if (reinterpret_cast<Handle*>(h)->precision == 64)
         return pair_contact<double>(*reinterpret_cast<Context<double>*>(reinterpret_cast<Handle*>(h)->ctx), ga, gb,
                                     pose_a, pose_b, mpr, out);
     return pair_contact<float>(*reinterpret_cast<Context<float>*>(reinterpret_cast<Handle*>(h)->ctx), ga, gb, pose_a,
                                pose_b, mpr, out);
+}
+
+// Operations counted so far by precision-0 handles (oracle_flops.h): out[0] all, out[1] library calls.
+void orc_flops(uint64_t* out) {
+    out[0] = FlopCounter::ops;
+    out[1] = FlopCounter::transcendental;
 }
 
 }  // extern "C"

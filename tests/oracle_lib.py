@@ -41,12 +41,22 @@ def load():
         lib.orc_pair_contact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         lib.orc_pair_contact.restype = ctypes.c_int
+        lib.orc_flops.argtypes = [ctypes.c_void_p]
+        lib.orc_flops.restype = None
         _lib = lib
     return _lib
 
 
+def flops() -> tuple[int, int]:
+    """Floating-point operations counted so far by precision-0 oracles (all, library calls; oracle/oracle_flops.h)."""
+    out = (ctypes.c_uint64 * 2)()
+    load().orc_flops(out)
+    return int(out[0]), int(out[1])
+
+
 class Oracle:
-    """CPU restatement of the rigid step for ``n_envs`` environments at FP32 or FP64."""
+    """CPU restatement of the rigid step for ``n_envs`` environments at FP32 or FP64, or (precision 0) in double
+    precision counting its floating-point operations (``flops()``)."""
 
     def __init__(self, model, n_envs: int, precision: int = 64):
         lib = load()
