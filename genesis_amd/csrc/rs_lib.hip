@@ -65,6 +65,7 @@ struct rs_model {
     int occupancy = 0;  // 0: compiler's register budget; 3 or 4: at least that many wavefronts per SIMD
     int layout = RS_LAYOUT_ENV_RECORDS;  // state layout of the per-environment (local-kind) fields
     int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
+    int n_cu = 0;       // compute units of the device
 };
 
 // Shared launches of several (model, slab) batches: a ring of pinned host + device segment tables, each reused only
@@ -407,6 +408,17 @@ __global__ void __launch_bounds__(kErrnoBlock) k_errno_final(const int32_t* part
 // ------------------------------------------------------------------------------------------------ launch helpers
 static inline dim3 env_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
+// Environments per workgroup of the narrowphase kernel (one lane per environment): the largest power of two up to 64
+// that still gives every compute unit four workgroups, so that a batch spreads over the whole device and each
+// wavefront waits on fewer environments' divergent detections (4096 environments on 256 CUs: 4 per workgroup). The
+// other per-stage kernels keep 64 environments per workgroup, whose loads of [row][env] planes coalesce.
+static inline int plain_block(const rs_model* m, int n_envs) {
+    const int n_cu = m->n_cu > 0 ? m->n_cu : 1;
+    int tpb = kBlock;
+    while (tpb > 1 && (n_envs + tpb - 1) / tpb < 4 * n_cu) tpb >>= 1;
+    return tpb;
+}
+
 static int launch_stage(const rs_model* m, const DState& st, int stage, hipStream_t s) {
     const DModel& dm = m->dm;
     if (stage == RS_STAGE_AABB) {
@@ -416,7 +428,8 @@ static int launch_stage(const rs_model* m, const DState& st, int stage, hipStrea
         // no candidate pairs: detection stops after the AABBs (collider.py:690-691)
     } else if ((stage == RS_STAGE_CONSTRAINTS || stage == RS_STAGE_SOLVE) && dm.disable_constraint) {
     } else {
-        dim3 g = env_grid(st.B), blk(kBlock);
+        const int tpb = stage == RS_STAGE_NARROWPHASE ? plain_block(m, st.B) : kBlock;
+        dim3 g((unsigned)((st.B + tpb - 1) / tpb)), blk((unsigned)tpb);
         switch (stage) {
             case RS_STAGE_FK: hipLaunchKernelGGL(k_stage<RS_STAGE_FK>, g, blk, 0, s, dm, st); break;
             case RS_STAGE_CRBA: hipLaunchKernelGGL(k_stage<RS_STAGE_CRBA>, g, blk, 0, s, dm, st); break;
@@ -577,6 +590,7 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device) != hipSuccess) max_lds = 0;
     m->max_lds = max_lds < 65536 ? max_lds : 65536;
+    if (hipDeviceGetAttribute(&m->n_cu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess) m->n_cu = 0;
     // default launch shape: two environments per workgroup, private regions in LDS when they fit
     m->fa.epb = 2;
     m->use_lds = check_fused_config(m, 2, 1) == RS_OK;
