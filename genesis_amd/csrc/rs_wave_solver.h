@@ -168,9 +168,69 @@ __device__ inline void cholesky_in_place(int D, float* H, float eps, int lane) {
     }
 }
 
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
+constexpr int kMfmaMaxDofs = 128;  // 8 x 8 tiles of 16
+
+// solver.py:1395-1452 (func_hessian_direct_tiled's product J^T diag(efc_D active) J) on the matrix cores, for models
+// above the row-tile classes (dense rows over many dofs): the lower triangle in 16 x 16 tiles, each accumulated over
+// the rows 4 at a time with v_mfma_f32_16x16x4f32 (A: lane l holds J[c0 + l/16][I*16 + l%16]; B: lane l holds
+// efc_D active J[c0 + l/16][J*16 + l%16]; tile element (4 (l/16) + r, l%16) in lane l's r-th accumulator). As in the
+// reference, a row's term of entry (i, j) is left out when |J[c][i]| <= EPS; the mass matrix is added afterwards on
+// blocks of one entity. The matrix cores sum the rows in their own order, so the entries differ from the
+// one-lane-per-entry sum (and from the per-stage kernels) in the last bits.
+__device__ inline void wave_hessian_mfma(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
+    const int D = m.D;
+    const int nc = LF(n_con, 0);
+    const int T = (D + 15) / 16;
+    const float eps = m.eps;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int I = 0; I < T; I++) {
+        mfma_v4f acc[kMfmaMaxDofs / 16];
+#pragma unroll
+        for (int J = 0; J < kMfmaMaxDofs / 16; J++) acc[J] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+        const int di = I * 16 + li;
+        for (int c0 = 0; c0 < nc; c0 += 4) {
+            const int c = c0 + lk;
+            const bool rv = c < nc;
+            const float w = rv ? R.efc_D(c) * (float)R.active(c) : 0.f;
+            float a = (rv && di < D) ? R.jac(c, di) : 0.f;
+            if (!(fabsf(a) > eps)) a = 0.f;
+#pragma unroll
+            for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
+                if (J <= I) {
+                    const int dj = J * 16 + li;
+                    const float bv = (rv && dj < D) ? R.jac(c, dj) * w : 0.f;
+                    acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[J], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
+            if (J <= I) {
+                const int j = J * 16 + li;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = I * 16 + 4 * lk + r;
+                    if (i < D && j <= i) {
+                        float h = acc[J][r];
+                        if (m.dof_entity[i] == m.dof_entity[j]) h = h + LF(mass_mat, i * D + j);
+                        LF(nt_H, i * D + j) = h;
+                    }
+                }
+            }
+        }
+    }
+    wave_sync();
+}
+
 // solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
 __device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
+    if (D > kTileWideDofs && D <= kMfmaMaxDofs) {
+        wave_hessian_mfma(m, loc, R, lane);
+        cholesky_in_place(D, FP(loc, nt_H), m.eps, lane);
+        return;
+    }
     const int nc = LF(n_con, 0);
     const int nE = D * (D + 1) / 2;
     const float eps = m.eps;

@@ -49,3 +49,28 @@ def scaled_err(a, b):
     if a.size == 0:
         return 0.0
     return float(np.max(np.abs(a - b)) / (np.max(np.abs(b)) + 1e-30))
+
+
+# Models above this many dofs assemble the fused solver's Hessian on the matrix cores (rs_wave_solver.h
+# wave_hessian_mfma), which sum the rows in their own order: their fused step equals the per-stage step to the solver's
+# tolerance, not to the last bit.
+MFMA_MIN_DOFS = 25
+
+
+def check_fused_matches_plain(plain, fused, model, names=("qpos", "vel", "acc", "n_contacts", "n_con")):
+    """After one substep from identical state: bit-equal for models below MFMA_MIN_DOFS; above it, the contact and
+    row counts equal (detection precedes the solve), qpos/vel within 5e-5 of their magnitude, and the constrained
+    acceleration within 2e-3 of max(|acc|, 1) per environment (the solver stops at its tolerance, solver.py:3063-3139:
+    two Hessians that differ in the last bits give iterates that differ up to it)."""
+    for name in names:
+        a, b = get_dev(plain, name), get_dev(fused, name)
+        if model.n_dofs < MFMA_MIN_DOFS or name in ("n_contacts", "n_con", "n_broad"):
+            d = float(np.max(np.abs(a - b))) if a.size else 0.0
+            assert d == 0.0, f"{name}: fused differs from per-stage by {d:.3e}"
+        elif name == "acc":
+            scale = np.maximum(np.abs(a).max(axis=1, keepdims=True), 1.0)
+            err = float(np.max(np.abs(b - a) / scale))
+            assert err < 2e-3, f"acc: {err:.3e}"
+        else:
+            err = scaled_err(b, a)
+            assert err < 5e-5, f"{name}: {err:.3e}"
