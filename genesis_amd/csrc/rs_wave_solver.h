@@ -189,19 +189,28 @@ __device__ inline void wave_hessian_mfma(const DModel& m, const LState& loc, con
 #pragma unroll
         for (int J = 0; J < kMfmaMaxDofs / 16; J++) acc[J] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
         const int di = I * 16 + li;
-        for (int c0 = 0; c0 < nc; c0 += 4) {
-            const int c = c0 + lk;
-            const bool rv = c < nc;
-            const float w = rv ? R.efc_D(c) * (float)R.active(c) : 0.f;
-            float a = (rv && di < D) ? R.jac(c, di) : 0.f;
-            if (!(fabsf(a) > eps)) a = 0.f;
+        // 16 rows per trip: the operands of four 4-row steps are loaded before their products, so that the loads of a
+        // trip are in flight together
+        for (int c0 = 0; c0 < nc; c0 += 16) {
+            float av[4], bv[4][kMfmaMaxDofs / 16];
 #pragma unroll
-            for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
-                if (J <= I) {
+            for (int s = 0; s < 4; s++) {
+                const int c = c0 + 4 * s + lk;
+                const bool rv = c < nc;
+                const float w = rv ? R.efc_D(c) * (float)R.active(c) : 0.f;
+                float a = (rv && di < D) ? R.jac(c, di) : 0.f;
+                av[s] = fabsf(a) > eps ? a : 0.f;
+#pragma unroll
+                for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
                     const int dj = J * 16 + li;
-                    const float bv = (rv && dj < D) ? R.jac(c, dj) * w : 0.f;
-                    acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[J], 0, 0, 0);
+                    bv[s][J] = (J <= I && rv && dj < D) ? R.jac(c, dj) * w : 0.f;
                 }
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+#pragma unroll
+                for (int J = 0; J < kMfmaMaxDofs / 16; J++)
+                    if (J <= I) acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s][J], acc[J], 0, 0, 0);
             }
         }
 #pragma unroll
@@ -223,12 +232,77 @@ __device__ inline void wave_hessian_mfma(const DModel& m, const LState& loc, con
     wave_sync();
 }
 
+// solver.py:1595-1621 in 16-column panels, for the models whose Hessian is assembled on the matrix cores: each panel's
+// columns are factored as in cholesky_in_place for every row at or below the panel, with the sums over k starting at
+// the panel's first column (the earlier panels' terms were subtracted already), then the panel's product
+// L[rows][panel] L[cols][panel]^T is subtracted from the trailing lower triangle in 16 x 16 tiles on the matrix cores.
+// The same factor as cholesky_in_place up to the order of the sums. H: D x D row-major, lower triangle, D <= 128.
+__device__ inline void cholesky_blocked_mfma(int D, float* H, float eps, int lane) {
+    const int T = (D + 15) / 16;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int p = 0; p < T; p++) {
+        const int k0 = p * 16, k1 = min(k0 + 16, D);
+        for (int i = k0; i < k1; i++) {
+            const float* Hi = H + (size_t)i * D;
+            float hv[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) hv[u] = k0 + u < i ? Hi[k0 + u] : 0.f;
+            float t = Hi[i];
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                if (k0 + u < i) t = t - hv[u] * hv[u];
+            const float dii = sqrtf(fmaxf(t, eps));
+            const float inv = 1.f / dii;
+            for (int r = i + 1 + lane; r < D; r += kWave) {
+                const float* Hr = H + (size_t)r * D;
+                float rv[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) rv[u] = k0 + u < i ? Hr[k0 + u] : 0.f;
+                float dt = 0.f;
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                    if (k0 + u < i) dt = dt + rv[u] * hv[u];
+                H[(size_t)r * D + i] = (Hr[i] - dt) * inv;
+            }
+            wave_sync();
+            H[(size_t)i * D + i] = dii;
+            wave_sync();
+        }
+        for (int Rt = p + 1; Rt < T; Rt++) {
+            for (int Ct = p + 1; Ct <= Rt; Ct++) {
+                const int ra = Rt * 16 + li, rb = Ct * 16 + li;
+                float a[4], b[4], h[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int k = k0 + 4 * kk + lk;
+                    a[kk] = (ra < D && k < k1) ? H[(size_t)ra * D + k] : 0.f;
+                    b[kk] = (rb < D && k < k1) ? H[(size_t)rb * D + k] : 0.f;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = Rt * 16 + 4 * lk + r;
+                    h[r] = (i < D && rb <= i) ? H[(size_t)i * D + rb] : 0.f;
+                }
+                mfma_v4f acc = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kk], b[kk], acc, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = Rt * 16 + 4 * lk + r;
+                    if (i < D && rb <= i) H[(size_t)i * D + rb] = h[r] - acc[r];
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
 // solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
 __device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
     if (D > kTileWideDofs && D <= kMfmaMaxDofs) {
         wave_hessian_mfma(m, loc, R, lane);
-        cholesky_in_place(D, FP(loc, nt_H), m.eps, lane);
+        cholesky_blocked_mfma(D, FP(loc, nt_H), m.eps, lane);
         return;
     }
     const int nc = LF(n_con, 0);
