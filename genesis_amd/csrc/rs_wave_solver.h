@@ -403,11 +403,27 @@ __device__ inline void wave_gradient_cg(const DModel& m, const LState& loc) {
     wave_sync();
 }
 
-// solver.py:2722-2751 + :1893-1910 for models with up to 128 dofs, lane-parallel: lane l holds rows l and l + 64.
-// Forward substitution column by column (row i subtracts L[i][j] x_j for j = 0, 1, ... in turn, the reference's
-// order for row i); backward substitution row by row, row i subtracting L[j][i] x_j for j = i + 1, i + 2, ... in
-// turn, with each product formed by lane j and the subtractions chained through readlane. The factor may live in
-// global memory (large models keep their private region there), so its columns are loaded kAhead steps early.
+// Sum over the 64 lanes with DPP (row_shr 1, 2, 4, 8 within each row of 16, then row_bcast:15 and row_bcast:31):
+// a source lane outside the row, or a row left out by the row mask, contributes 0; lane 63 ends with the sum.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dpp_zero_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_f(float x) {
+    x = x + dpp_zero_f<0x111>(x);
+    x = x + dpp_zero_f<0x112>(x);
+    x = x + dpp_zero_f<0x114>(x);
+    x = x + dpp_zero_f<0x118>(x);
+    x = x + dpp_zero_f<0x142, 0xa>(x);
+    x = x + dpp_zero_f<0x143, 0xc>(x);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), kWave - 1));
+}
+
+// solver.py:2722-2751 + :1893-1910 for models with 25 to 128 dofs (whose factor comes from the matrix cores),
+// lane-parallel: lane l holds rows l and l + 64. Forward substitution column by column (row i subtracts L[i][j] x_j for
+// j = 0, 1, ... in turn, the reference's order for row i); backward substitution row by row, row i's sum of
+// L[j][i] x_j over j > i formed by lanes j and reduced across the wavefront. The factor may live in global memory
+// (large models keep their private region there), so its columns are loaded kAhead steps early.
 __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, int lane) {
     constexpr int kAhead = 4;
     const int D = m.D;
@@ -452,7 +468,9 @@ __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, i
             }
         }
     }
-    // backward: pb[k][r] = L[row r of this lane][column i], for i = D - 1 - (ii0 + k)
+    // backward: pb[k][r] = L[row r of this lane][column i], for i = D - 1 - (ii0 + k). Row i's sum over j > i of
+    // L[j][i] x_j is a wavefront reduction (a tree, not the reference's order j = i + 1, i + 2, ...: these models'
+    // factor comes from the matrix cores anyway), so each row costs a reduction instead of a chain of D - i readlanes.
     float pb[kAhead][2];
 #pragma unroll
     for (int k = 0; k < kAhead; k++)
@@ -463,36 +481,12 @@ __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, i
         for (int k = 0; k < kAhead; k++) {
             const int i = D - 1 - (ii0 + k);
             if (i >= 0) {
-                float prod[2];
+                // lane j: L[j][i] x_j (zero unless j > i)
+                const float prod = pb[k][0] * x[0] + pb[k][1] * x[1];
 #pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    prod[r] = pb[k][r] * x[r];  // lane j: L[j][i] x_j (zero unless j > i)
-                    pb[k][r] = colv(lane + r * kWave, i - kAhead);
-                }
+                for (int r = 0; r < 2; r++) pb[k][r] = colv(lane + r * kWave, i - kAhead);
                 const int ri = i >> 6, li = i & (kWave - 1);
-                float c = readlane_f(ri ? x[1] : x[0], li);
-                // j = i + 1 .. D - 1 in order: first the rows held in prod[0] (j < 64), then those in prod[1]
-                const int j1 = min(D, kWave);
-                int j = i + 1;
-                for (; j + 4 <= j1; j += 4) {
-                    const float t0 = readlane_f(prod[0], j), t1 = readlane_f(prod[0], j + 1);
-                    const float t2 = readlane_f(prod[0], j + 2), t3 = readlane_f(prod[0], j + 3);
-                    c = c - t0;
-                    c = c - t1;
-                    c = c - t2;
-                    c = c - t3;
-                }
-                for (; j < j1; j++) c = c - readlane_f(prod[0], j);
-                j = max(j, kWave);
-                for (; j + 4 <= D; j += 4) {
-                    const float t0 = readlane_f(prod[1], j - kWave), t1 = readlane_f(prod[1], j - kWave + 1);
-                    const float t2 = readlane_f(prod[1], j - kWave + 2), t3 = readlane_f(prod[1], j - kWave + 3);
-                    c = c - t0;
-                    c = c - t1;
-                    c = c - t2;
-                    c = c - t3;
-                }
-                for (; j < D; j++) c = c - readlane_f(prod[1], j - kWave);
+                float c = readlane_f(ri ? x[1] : x[0], li) - wave_sum_f(prod);
                 c = c / readlane_f(ri ? hd[1] : hd[0], li);
                 if (lane == li) {
                     if (ri) x[1] = c;
