@@ -80,7 +80,98 @@ __device__ inline void wave_constraints(const DModel& m, const LState& loc, cons
     wave_sync();
 }
 
-// solver.py:2620-2694 (same arithmetic as dev_update_constraint)
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
+constexpr int kMfmaMaxDofs = 128;  // 8 x 8 tiles of 16
+// models above the row-tile classes run their dense row products on the matrix cores
+__device__ __forceinline__ bool mfma_model(int D) { return D > kTileWideDofs && D <= kMfmaMaxDofs; }
+
+// Sum over the 64 lanes with DPP (row_shr 1, 2, 4, 8 within each row of 16, then row_bcast:15 and row_bcast:31):
+// a source lane outside the row, or a row left out by the row mask, contributes 0; lane 63 ends with the sum.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ float dpp_zero_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_f(float x) {
+    x = x + dpp_zero_f<0x111>(x);
+    x = x + dpp_zero_f<0x112>(x);
+    x = x + dpp_zero_f<0x114>(x);
+    x = x + dpp_zero_f<0x118>(x);
+    x = x + dpp_zero_f<0x142, 0xa>(x);
+    x = x + dpp_zero_f<0x143, 0xc>(x);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), kWave - 1));
+}
+
+// out[d] = sum over rows c < nc of J[c][d] w[c] (d < D <= 128) on the matrix cores: for each 16-column tile, A holds
+// the row weights (A[i][k] = w[c0 + k] for every i) and B the rows (B[k][j] = J[c0 + k][16 T + j]), so every row of
+// the tile's result is the sum; 16 rows per trip, their operands loaded before the products.
+template <class W>
+__device__ inline void gemv_t_mfma(const WorkRows& R, int nc, int D, float* out, int lane, W w) {
+    const int li = lane & 15, lk = lane >> 4;
+    mfma_v4f acc[kMfmaMaxDofs / 16];
+#pragma unroll
+    for (int t = 0; t < kMfmaMaxDofs / 16; t++) acc[t] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < nc; c0 += 16) {
+        float av[4], bv[4][kMfmaMaxDofs / 16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = c0 + 4 * q + lk;
+            av[q] = c < nc ? w(c) : 0.f;
+#pragma unroll
+            for (int t = 0; t < kMfmaMaxDofs / 16; t++) {
+                const int d = t * 16 + li;
+                bv[q][t] = (c < nc && d < D) ? R.jac(c, d) : 0.f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int t = 0; t < kMfmaMaxDofs / 16; t++)
+                if (t * 16 < D) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], bv[q][t], acc[t], 0, 0, 0);
+    }
+    if (lk == 0) {
+#pragma unroll
+        for (int t = 0; t < kMfmaMaxDofs / 16; t++)
+            if (t * 16 + li < D) out[t * 16 + li] = acc[t][0];
+    }
+    wave_sync();
+}
+
+// R.jv(c) = sum over d of J[c][d] v[d] for c < nc (D <= 128) on the matrix cores: 16-row tiles, four at a time, A the
+// rows (A[i][k] = J[c0 + i][d0 + k]) and B the vector over the tile's columns (B[k][j] = v[d0 + k]), so every column
+// of a tile's result is the row sums.
+__device__ inline void gemv_rows_mfma(const WorkRows& R, int nc, int D, const float* v, int lane) {
+    const int li = lane & 15, lk = lane >> 4;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        mfma_v4f acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+        for (int d0 = 0; d0 < D; d0 += 4) {
+            const int d = d0 + lk;
+            const float b = d < D ? v[d] : 0.f;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int c = c0 + 16 * q + li;
+                a[q] = (c < nc && d < D) ? R.jac(c, d) : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b, acc[q], 0, 0, 0);
+        }
+        if (li == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int c = c0 + 16 * q + 4 * lk + r;
+                    if (c < nc) R.jv(c) = acc[q][r];
+                }
+        }
+    }
+    wave_sync();
+}
+
+// solver.py:2620-2694 (same arithmetic as dev_update_constraint; for the matrix-core models J^T force on the matrix
+// cores and the cost's row sums reduced across the wavefront)
 __device__ inline float wave_update_constraint(const DModel& m, const LState& loc, const WorkRows& R, int lane,
                                                float cost_in) {
     const int D = m.D;
@@ -109,6 +200,25 @@ __device__ inline float wave_update_constraint(const DModel& m, const LState& lo
         R.tmp(1, c) = 0.5f * (ja * ja * Dc * (float)act);
     }
     wave_sync();
+    if (mfma_model(D)) {
+        gemv_t_mfma(R, nc, D, FP(loc, qfrc_constraint), lane, [&](int c) { return R.force(c); });
+        float pf = 0.f, pq = 0.f;
+        for (int c = lane; c < nc; c += kWave) {
+            if (ne <= c && c < nef) pf = pf + R.tmp(0, c);
+            pq = pq + R.tmp(1, c);
+        }
+        float cost = wave_sum_f(pf);
+        float gauss = 0.f;
+        for (int d = 0; d < D; d++) {
+            float v = 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
+            gauss = gauss + v;
+            cost = cost + v;
+        }
+        cost = cost + wave_sum_f(pq);
+        LF(gauss, 0) = gauss;
+        wave_sync();
+        return cost;
+    }
     for (int d = lane; d < D; d += kWave)
         LF(qfrc_constraint, d) = seq_sum(0.f, 0, nc, [&](int c) { return R.jac(c, d) * R.force(c); });
     // uniform: the cost in the reference's order (friction-loss terms, Gauss terms, quadratic row terms)
@@ -167,9 +277,6 @@ __device__ inline void cholesky_in_place(int D, float* H, float eps, int lane) {
         wave_sync();
     }
 }
-
-typedef float mfma_v4f __attribute__((ext_vector_type(4)));
-constexpr int kMfmaMaxDofs = 128;  // 8 x 8 tiles of 16
 
 // solver.py:1395-1452 (func_hessian_direct_tiled's product J^T diag(efc_D active) J) on the matrix cores, for models
 // above the row-tile classes (dense rows over many dofs): the lower triangle in 16 x 16 tiles, each accumulated over
@@ -300,7 +407,7 @@ __device__ inline void cholesky_blocked_mfma(int D, float* H, float eps, int lan
 // solver.py:1395-1452 + :1595-1621: H = M + J^T diag(D active) J (lower), then in-place Cholesky
 __device__ inline void wave_hessian_cholesky(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
     const int D = m.D;
-    if (D > kTileWideDofs && D <= kMfmaMaxDofs) {
+    if (mfma_model(D)) {
         wave_hessian_mfma(m, loc, R, lane);
         cholesky_blocked_mfma(D, FP(loc, nt_H), m.eps, lane);
         return;
@@ -401,22 +508,6 @@ __device__ inline void wave_gradient_cg(const DModel& m, const LState& loc) {
     for (int d = 0; d < D; d++) LF(grad, d) = LF(Ma, d) - LF(force, d) - LF(qfrc_constraint, d);
     dev_solve_mass(m, loc, 0, FV(loc, grad, 0), FV(loc, Mgrad, 0));
     wave_sync();
-}
-
-// Sum over the 64 lanes with DPP (row_shr 1, 2, 4, 8 within each row of 16, then row_bcast:15 and row_bcast:31):
-// a source lane outside the row, or a row left out by the row mask, contributes 0; lane 63 ends with the sum.
-template <int kCtrl, int kRowMask = 0xf>
-__device__ __forceinline__ float dpp_zero_f(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, kRowMask, 0xf, false));
-}
-__device__ __forceinline__ float wave_sum_f(float x) {
-    x = x + dpp_zero_f<0x111>(x);
-    x = x + dpp_zero_f<0x112>(x);
-    x = x + dpp_zero_f<0x114>(x);
-    x = x + dpp_zero_f<0x118>(x);
-    x = x + dpp_zero_f<0x142, 0xa>(x);
-    x = x + dpp_zero_f<0x143, 0xc>(x);
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), kWave - 1));
 }
 
 // solver.py:2722-2751 + :1893-1910 for models with 25 to 128 dofs (whose factor comes from the matrix cores),
@@ -854,6 +945,9 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
     LF(quad_gauss, 0) = gauss;
     LF(quad_gauss, 1) = g1;
     LF(quad_gauss, 2) = g2;
+    // the matrix-core models take J search from the matrix cores
+    const bool jv_mfma = KD == 0 && mfma_model(D);
+    if (jv_mfma) gemv_rows_mfma(R, nc, D, FP(loc, search), lane);
     // lanes 0-2 sum t[q] = (gauss, g1, g2)[q] + f[q] over all rows, lanes 3-5 e[q] over the equality rows, in row
     // order, from tile row q
     float acc = lane == 0 ? gauss : lane == 1 ? g1 : lane == 2 ? g2 : 0.f;
@@ -886,6 +980,8 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
 #pragma unroll
                 for (int d = 0; d < KD; d++)
                     if (d < D) jv = jv + jr[d] * sr[d];
+            } else if (jv_mfma) {
+                jv = R.jv(c);
             } else {
                 for (int d = 0; d < D; d++) jv = jv + R.jac(c, d) * LF(search, d);
             }
