@@ -387,9 +387,12 @@ static __global__ void k_sched_flip(DState st) {
 }
 
 // kOcc > 0 asks the compiler for at least kOcc wavefronts per SIMD (fewer registers, more spills); 0 leaves the
-// register budget to the compiler. The autotuner picks between them.
+// register budget to the compiler. The autotuner picks between them. Models above 24 dofs (kPath 0) run at most
+// kGenericMaxThreads lanes (4 environments) per workgroup, so that a wavefront may hold all 512 registers of its SIMD
+// (the matrix-core Hessian keeps its 36 tile accumulators resident).
 template <bool kLds, int kOcc, int kPath>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kOcc > 0 ? kOcc : 1, 8)))
+__global__ void __launch_bounds__(kPath == 0 ? kGenericMaxThreads : 512)
+__attribute__((amdgpu_waves_per_eu(kOcc > 0 ? kOcc : 1, 8)))
 k_fused_wave(DModel m, DState st, FusedArgs fa) {
     extern __shared__ float lds_env[];
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));

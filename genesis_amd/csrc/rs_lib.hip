@@ -468,6 +468,8 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
 
 static int check_fused_config(const rs_model* m, int epb, int use_lds) {
     if (epb < 1 || epb > 8) return fail(RS_E_INVALID, "fused config: envs_per_block must be in 1..8");
+    if (!m->fa.tiled && epb * kWave > kGenericMaxThreads)
+        return fail(RS_E_UNSUPPORTED, "fused config: models above 24 dofs run at most 4 environments per workgroup");
     if ((int64_t)epb * lds_slice_floats(m->h, m->fa, use_lds) * 4 > m->max_lds)
         return fail(RS_E_UNSUPPORTED, use_lds ? "fused config: private regions do not fit in LDS at this envs_per_block"
                                               : "fused config: row tiles do not fit in LDS at this envs_per_block");

@@ -40,6 +40,7 @@ struct FusedArgs {
 constexpr int kTileStride = 68;
 constexpr int kTileMaxDofs = 16;   // row-tile solver variants for models with up to 16 dofs...
 constexpr int kTileWideDofs = 24;  // ... and a second class for 17 to 24 dofs (wider register rows)
+constexpr int kGenericMaxThreads = 256;  // lanes per workgroup of the fused kernel for larger models
 constexpr int kTileMinRows = 12;   // the ordered sums of the line search use up to 9 rows
 __host__ __device__ constexpr int tile_rows(int n_dofs) {
     return n_dofs <= kTileWideDofs && n_dofs + 3 > kTileMinRows ? n_dofs + 3 : kTileMinRows;

@@ -279,59 +279,68 @@ __device__ inline void cholesky_in_place(int D, float* H, float eps, int lane) {
 }
 
 // solver.py:1395-1452 (func_hessian_direct_tiled's product J^T diag(efc_D active) J) on the matrix cores, for models
-// above the row-tile classes (dense rows over many dofs): the lower triangle in 16 x 16 tiles, each accumulated over
-// the rows 4 at a time with v_mfma_f32_16x16x4f32 (A: lane l holds J[c0 + l/16][I*16 + l%16]; B: lane l holds
+// above the row-tile classes (dense rows over many dofs): the lower triangle in 16 x 16 tiles, all accumulated
+// together over the rows 4 at a time with v_mfma_f32_16x16x4f32 (A: lane l holds J[c0 + l/16][I*16 + l%16]; B: lane l holds
 // efc_D active J[c0 + l/16][J*16 + l%16]; tile element (4 (l/16) + r, l%16) in lane l's r-th accumulator). As in the
 // reference, a row's term of entry (i, j) is left out when |J[c][i]| <= EPS; the mass matrix is added afterwards on
 // blocks of one entity. The matrix cores sum the rows in their own order, so the entries differ from the
 // one-lane-per-entry sum (and from the per-stage kernels) in the last bits.
 __device__ inline void wave_hessian_mfma(const DModel& m, const LState& loc, const WorkRows& R, int lane) {
+    constexpr int kT = kMfmaMaxDofs / 16;
+    constexpr int kTiles = kT * (kT + 1) / 2;  // the lower triangle's 16 x 16 tiles, (I, J <= I) in row order
     const int D = m.D;
     const int nc = LF(n_con, 0);
     const int T = (D + 15) / 16;
     const float eps = m.eps;
     const int li = lane & 15, lk = lane >> 4;
-    for (int I = 0; I < T; I++) {
-        mfma_v4f acc[kMfmaMaxDofs / 16];
+    // A (lane: row i = li of tile row I, k = lk) and B (lane: k = lk, column j = li of tile column J) read the same
+    // element J[c0 + lk][16 I + li], so each 4-row step loads every row's 16-column slices once and feeds all the
+    // tiles from them; two steps per trip keep 16 loads in flight.
+    mfma_v4f acc[kTiles];
 #pragma unroll
-        for (int J = 0; J < kMfmaMaxDofs / 16; J++) acc[J] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
-        const int di = I * 16 + li;
-        // 16 rows per trip: the operands of four 4-row steps are loaded before their products, so that the loads of a
-        // trip are in flight together
-        for (int c0 = 0; c0 < nc; c0 += 16) {
-            float av[4], bv[4][kMfmaMaxDofs / 16];
+    for (int t = 0; t < kTiles; t++) acc[t] = mfma_v4f{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < nc; c0 += 8) {
+        float v[2][kT], w[2];
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int c = c0 + 4 * s + lk;
-                const bool rv = c < nc;
-                const float w = rv ? R.efc_D(c) * (float)R.active(c) : 0.f;
-                float a = (rv && di < D) ? R.jac(c, di) : 0.f;
-                av[s] = fabsf(a) > eps ? a : 0.f;
+        for (int s = 0; s < 2; s++) {
+            const int c = c0 + 4 * s + lk;
+            const bool rv = c < nc;
+            w[s] = rv ? R.efc_D(c) * (float)R.active(c) : 0.f;
 #pragma unroll
-                for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
-                    const int dj = J * 16 + li;
-                    bv[s][J] = (J <= I && rv && dj < D) ? R.jac(c, dj) * w : 0.f;
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-#pragma unroll
-                for (int J = 0; J < kMfmaMaxDofs / 16; J++)
-                    if (J <= I) acc[J] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s][J], acc[J], 0, 0, 0);
+            for (int I = 0; I < kT; I++) {
+                const int d = I * 16 + li;
+                v[s][I] = (rv && d < D) ? R.jac(c, d) : 0.f;
             }
         }
 #pragma unroll
-        for (int J = 0; J < kMfmaMaxDofs / 16; J++) {
-            if (J <= I) {
-                const int j = J * 16 + li;
+        for (int s = 0; s < 2; s++) {
+            float av[kT], bv[kT];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int i = I * 16 + 4 * lk + r;
-                    if (i < D && j <= i) {
-                        float h = acc[J][r];
-                        if (m.dof_entity[i] == m.dof_entity[j]) h = h + LF(mass_mat, i * D + j);
-                        LF(nt_H, i * D + j) = h;
-                    }
+            for (int I = 0; I < kT; I++) {
+                av[I] = fabsf(v[s][I]) > eps ? v[s][I] : 0.f;
+                bv[I] = v[s][I] * w[s];
+            }
+#pragma unroll
+            for (int I = 0; I < kT; I++)
+#pragma unroll
+                for (int J = 0; J <= I; J++)
+                    if (I < T) acc[I * (I + 1) / 2 + J] =
+                        __builtin_amdgcn_mfma_f32_16x16x4f32(av[I], bv[J], acc[I * (I + 1) / 2 + J], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int I = 0; I < kT; I++) {
+#pragma unroll
+        for (int J = 0; J <= I; J++) {
+            if (I >= T) continue;
+            const int j = J * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = I * 16 + 4 * lk + r;
+                if (i < D && j <= i) {
+                    float h = acc[I * (I + 1) / 2 + J][r];
+                    if (m.dof_entity[i] == m.dof_entity[j]) h = h + LF(mass_mat, i * D + j);
+                    LF(nt_H, i * D + j) = h;
                 }
             }
         }
