@@ -6,6 +6,14 @@ reached. Every tick advances every admitted request by one substep in a single s
 (``runtime.SharedLaunch`` -> ``rs_batch_step``), so requests of different scenes and sizes overlap on the device; a
 finished request releases its memory and slot immediately, which admits the next waiting ones. Each environment's
 trajectory is the same as when its slab is stepped alone (``tests/test_gpu_serving.py`` checks this bit for bit).
+
+Preemption and request checkpoints (after ``scene.py:1402-1446``, the reference's whole-scene save/load, here per
+request): a waiting request of higher ``priority`` that does not fit evicts active requests of lower priority (lowest
+first, the latest admitted first among equals), but only when evicting them makes it fit. An evicted request's state
+slab is copied whole into pinned host memory and its device memory released; it waits again, and on re-admission
+its slab is restored from that copy and continues from the substep it reached, bit for bit. ``save_request`` writes
+a request (its description, progress and, once started, its state) to a file from which ``load_request`` rebuilds
+it, so a request can also move between servers or processes.
 """
 
 from __future__ import annotations
@@ -37,6 +45,9 @@ class Request:
     nbytes: int = 0
     final_qpos: torch.Tensor | None = None
     final_vel: torch.Tensor | None = None
+    priority: int = 0
+    preemptions: int = 0
+    host_state: torch.Tensor | None = None  # the slab of a started request that is not resident (pinned host memory)
 
 
 def poisson_requests(n: int, rate_hz: float, seed: int = 4, min_envs: int = 64, max_envs: int = 4096,
@@ -57,9 +68,12 @@ def poisson_requests(n: int, rate_hz: float, seed: int = 4, min_envs: int = 64, 
 class RolloutServer:
     """Admission by HBM budget, shared fused launches, per-tick latency accounting."""
 
-    def __init__(self, hbm_budget_bytes: int, max_active: int = 32, stream: torch.cuda.Stream | None = None):
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 32, stream: torch.cuda.Stream | None = None,
+                 preempt: bool = True):
         self.budget = int(hbm_budget_bytes)
         self.max_active = max_active
+        self.preempt = preempt
+        self.preemptions = 0
         self.stream = stream or torch.cuda.Stream()
         self.shared = rt.SharedLaunch(max_active)
         self._models: dict[str, rt.DeviceModel] = {}
@@ -101,39 +115,141 @@ class RolloutServer:
         self.init_state(req, slab)
         return slab
 
+    # ------------------------------------------------------------------ residency
+    def admit(self, req: Request) -> None:
+        """Give a request its device slab: restored from its host copy if it was started, else its initial state."""
+        if req.slab is not None:
+            raise ValueError(f"request {req.rid} is already resident")
+        if not req.nbytes:
+            req.nbytes = self.slab_bytes(req.scene, req.n_envs)
+        with torch.cuda.stream(self.stream):
+            if req.host_state is not None:
+                slab = rt.StateSlab(self.model(req.scene), req.n_envs)
+                if slab.nbytes != req.host_state.numel():
+                    raise ValueError(f"request {req.rid}: saved state holds {req.host_state.numel()} bytes, its slab "
+                                     f"{slab.nbytes}")
+                slab.buffer.copy_(req.host_state)
+                req.host_state = None
+            else:
+                slab = self.new_slab(req)
+        req.slab = slab
+        self.used += req.nbytes
+
+    def evict(self, req: Request) -> None:
+        """Copy a resident request's slab whole into pinned host memory and release its device memory."""
+        if req.slab is None:
+            raise ValueError(f"request {req.rid} is not resident")
+        with torch.cuda.stream(self.stream):
+            host = torch.empty(req.slab.nbytes, dtype=torch.uint8, pin_memory=True)
+            host.copy_(req.slab.buffer)  # synchronous: the copy has landed when it returns
+        req.host_state = host
+        req.slab = None
+        req.preemptions += 1
+        self.preemptions += 1
+        self.used -= req.nbytes
+
+    def tick(self, active: list[Request]) -> float:
+        """Advance every request in ``active`` by one substep in one shared launch; the launch's time in ms."""
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(self.stream):
+            e0.record(self.stream)
+            self.shared.step([r.slab for r in active], self.stream)
+            e1.record(self.stream)
+        e1.synchronize()
+        for r in active:
+            r.steps_done += 1
+        return e0.elapsed_time(e1)
+
+    # ------------------------------------------------------------------ checkpoints
+    def save_request(self, req: Request, path) -> None:
+        """A request's description, progress and (once started) full state to one ``.npz`` file."""
+        meta = np.array([req.rid, req.n_envs, req.horizon, req.steps_done, req.priority], dtype=np.int64)
+        arrays = {"meta": meta, "scene": np.array(req.scene), "arrival": np.array(req.arrival, dtype=np.float64)}
+        if req.slab is not None:
+            torch.cuda.current_stream().wait_stream(self.stream)
+            arrays["state"] = req.slab.buffer.cpu().numpy()
+        elif req.host_state is not None:
+            arrays["state"] = req.host_state.numpy()
+        with open(path, "wb") as f:
+            np.savez(f, **arrays)
+
+    def load_request(self, path) -> Request:
+        """Rebuild a request written by :meth:`save_request`; a started one resumes from its saved substep."""
+        with np.load(path, allow_pickle=False) as z:
+            rid, n_envs, horizon, steps_done, priority = (int(x) for x in z["meta"])
+            scene = str(z["scene"])
+            if scene not in SCENES:
+                raise ValueError(f"{path!s}: unknown scene {scene!r}")
+            req = Request(rid, scene, n_envs, horizon, float(z["arrival"]), steps_done=steps_done, priority=priority)
+            req.nbytes = self.slab_bytes(scene, n_envs)
+            if "state" in z.files:
+                state = z["state"]
+                if state.dtype != np.uint8 or state.size != req.nbytes:
+                    raise ValueError(f"{path!s}: state of {state.size} bytes, a {scene} slab of {n_envs} "
+                                     f"environments holds {req.nbytes}")
+                host = torch.empty(req.nbytes, dtype=torch.uint8, pin_memory=True)
+                host.copy_(torch.from_numpy(state))
+                req.host_state = host
+            elif steps_done:
+                raise ValueError(f"{path!s}: a started request without its state")
+        return req
+
+    def _victims(self, head: Request, active: list[Request]) -> list[Request] | None:
+        """Active requests of lower priority whose eviction lets ``head`` in (lowest priority first, the latest
+        admitted first among equals), or None when evicting all of them would not."""
+        cands = sorted((r for r in active if r.priority < head.priority),
+                       key=lambda r: (r.priority, -(r.admitted or 0.0)))
+        out, used, n = [], self.used, len(active)
+        for r in cands:
+            if n < self.max_active and used + head.nbytes <= self.budget:
+                break
+            out.append(r)
+            used -= r.nbytes
+            n -= 1
+        return out if (n < self.max_active and used + head.nbytes <= self.budget) else None
+
     # ------------------------------------------------------------------ loop
     def run(self, requests: list[Request], keep_final: bool = False) -> dict:
         waiting = sorted(requests, key=lambda r: r.arrival)
+        start = {id(r): r.steps_done for r in waiting}  # requests loaded from a checkpoint resume part-way
         for r in waiting:
             r.nbytes = self.slab_bytes(r.scene, r.n_envs)
             if r.nbytes > self.budget:
                 raise ValueError(f"request {r.rid} needs {r.nbytes} bytes, more than the budget {self.budget}")
+            if r.steps_done >= r.horizon or r.slab is not None:
+                raise ValueError(f"request {r.rid} is finished or already resident")
         active: list[Request] = []
         done: list[Request] = []
         t0 = time.perf_counter()
         with torch.cuda.stream(self.stream):
             while waiting or active:
                 now = time.perf_counter() - t0
-                while waiting and waiting[0].arrival <= now and len(active) < self.max_active \
-                        and self.used + waiting[0].nbytes <= self.budget:
-                    r = waiting.pop(0)
-                    r.slab = self.new_slab(r)
-                    r.admitted = time.perf_counter() - t0
-                    self.used += r.nbytes
-                    active.append(r)
+                while True:
+                    # the arrived request of highest priority, the earliest among equals
+                    arrived = [r for r in waiting if r.arrival <= now]
+                    if not arrived:
+                        break
+                    head = min(arrived, key=lambda r: (-r.priority, r.arrival, r.rid))
+                    if len(active) >= self.max_active or self.used + head.nbytes > self.budget:
+                        victims = self._victims(head, active) if self.preempt else None
+                        if not victims:
+                            break
+                        for v in victims:
+                            self.evict(v)
+                            active.remove(v)
+                            waiting.append(v)
+                    waiting.remove(head)
+                    self.admit(head)
+                    if head.admitted is None:
+                        head.admitted = time.perf_counter() - t0
+                    active.append(head)
                 if not active:
-                    time.sleep(max(0.0, min(waiting[0].arrival - now, 0.01)))
+                    time.sleep(max(0.0, min(min(r.arrival for r in waiting) - now, 0.01)))
                     continue
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(self.stream)
-                self.shared.step([r.slab for r in active], self.stream)
-                e1.record(self.stream)
-                e1.synchronize()
-                self.tick_ms.append(e0.elapsed_time(e1))
+                self.tick_ms.append(self.tick(active))
                 self.active_per_tick.append(len(active))
                 still = []
                 for r in active:
-                    r.steps_done += 1
                     if r.steps_done >= r.horizon:
                         r.finished = time.perf_counter() - t0
                         if keep_final:
@@ -146,7 +262,7 @@ class RolloutServer:
                         still.append(r)
                 active = still
         wall = time.perf_counter() - t0
-        env_steps = sum(r.n_envs * r.horizon for r in done)
+        env_steps = sum(r.n_envs * (r.horizon - start[id(r)]) for r in done)
         busy = sum(self.tick_ms) * 1e-3
         return {
             "requests": len(done),
@@ -160,4 +276,5 @@ class RolloutServer:
             "mean_active": float(np.mean(self.active_per_tick)),
             "max_active": int(np.max(self.active_per_tick)),
             "mean_queue_s": float(np.mean([r.admitted - r.arrival for r in done])),
+            "preemptions": self.preemptions,
         }

@@ -1,7 +1,8 @@
 """Serving loop: requests of different scenes, sizes and horizons co-batched into shared fused launches.
 
 Each request's result must be the same alone and in the mix: the environments are independent and the fused step is
-deterministic, so the check is bit-exact.
+deterministic, so the check is bit-exact. The same holds for a request that is evicted to host memory and restored
+(preemption), or saved to a file and rebuilt from it (request checkpoints, after scene.py:1402-1446).
 """
 
 from __future__ import annotations
@@ -81,3 +82,68 @@ def test_serving_mix_at_its_stated_size():
         print(f"[serving] request {r.rid} ({r.scene}, {r.n_envs} envs, {r.horizon} steps): alone == mixed; 5 steps vs "
               f"restatement: p99 {np.percentile(d, 99):.2e} max {d.max():.2e}", flush=True)
         assert np.percentile(d, 99) <= 1e-3 and d.max() <= 5e-2, r.rid
+
+
+def _alone_like(server, req):
+    return _alone(server, Request(req.rid, req.scene, req.n_envs, req.horizon, 0.0))
+
+
+def test_preemption_evicts_lower_priority_and_resumes_bit_exact():
+    """Two long requests of priority 0 fill the budget; a priority-1 request arriving later does not fit, evicts the
+    later-admitted of them to pinned host memory, runs, and the evicted one resumes when memory frees. Every request
+    ends bit-equal to its run alone."""
+    lo = [Request(0, "franka", 256, 600, 0.0), Request(1, "quadruped", 256, 600, 0.0)]
+    hi = Request(2, "boxpile", 64, 40, 0.05, priority=1)
+    probe = RolloutServer(hbm_budget_bytes=1 << 40)
+    need = [probe.slab_bytes(r.scene, r.n_envs) for r in lo + [hi]]
+    server = RolloutServer(hbm_budget_bytes=sum(need) - 1, max_active=8)  # the three never fit together
+    print(f"[serving] preemption run, slabs {need} bytes", flush=True)
+    stats = server.run(lo + [hi], keep_final=True)
+    print(f"[serving] preemptions {stats['preemptions']}, per request {[r.preemptions for r in lo + [hi]]}", flush=True)
+    assert stats["requests"] == 3 and server.used == 0
+    assert stats["preemptions"] >= 1 and hi.preemptions == 0 and lo[0].preemptions + lo[1].preemptions >= 1
+    assert hi.finished < max(lo[0].finished, lo[1].finished)
+    for r in lo + [hi]:
+        q, v = _alone_like(server, r)
+        assert torch.equal(q, r.final_qpos) and torch.equal(v, r.final_vel), f"request {r.rid} differs from alone"
+
+
+def test_request_eviction_and_file_checkpoint_round_trip(tmp_path):
+    """A request stepped 25 substeps, evicted and restored, stepped 10 more, saved to a file and rebuilt from it as a
+    new request object, then finished: bit-equal to its run alone. A file for another slab size is refused."""
+    import os
+
+    server = RolloutServer(hbm_budget_bytes=64 << 30)
+    r = Request(7, "quadruped", 128, 60, 0.0)
+    server.admit(r)
+    for _ in range(25):
+        server.tick([r])
+    server.evict(r)
+    assert r.slab is None and r.host_state is not None and server.used == 0
+    server.admit(r)
+    for _ in range(10):
+        server.tick([r])
+    path = os.path.join(str(tmp_path), "req7.npz")
+    server.save_request(r, path)
+    r2 = server.load_request(path)
+    assert (r2.rid, r2.scene, r2.n_envs, r2.horizon, r2.steps_done) == (7, "quadruped", 128, 60, 35)
+    server.admit(r2)
+    while r2.steps_done < r2.horizon:
+        server.tick([r2])
+    q, v = _alone_like(server, r)
+    assert torch.equal(q, r2.slab["qpos"]) and torch.equal(v, r2.slab["vel"])
+    # a request that has not started round-trips without state; a state of the wrong size is refused
+    fresh = Request(8, "franka", 32, 10, 0.0)
+    p2 = os.path.join(str(tmp_path), "req8.npz")
+    server.save_request(fresh, p2)
+    back = server.load_request(p2)
+    assert back.host_state is None and back.steps_done == 0
+    with np.load(path, allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files}
+    arrays["meta"] = arrays["meta"].copy()
+    arrays["meta"][1] = 64  # 64 environments: another slab size
+    p3 = os.path.join(str(tmp_path), "bad.npz")
+    with open(p3, "wb") as f:
+        np.savez(f, **arrays)
+    with pytest.raises(ValueError):
+        server.load_request(p3)
