@@ -63,6 +63,18 @@ def kernel_stats(directory: str) -> list[dict]:
     return out
 
 
+def dispatch_avg_us(directory: str, kernel: str, skip: int, count: int) -> tuple[float, int]:
+    """Mean duration (us) of dispatches skip .. skip + count - 1, in start order, of the kernels whose name contains
+    ``kernel``, from the per-dispatch table (``*_kernel_trace.csv``) of the same run; and how many were averaged."""
+    rows = [r for r in _rows(directory, "kernel_trace.csv") if kernel in r["kernel_name"]]
+    rows.sort(key=lambda r: int(r["start_timestamp"]))
+    sel = rows[skip: skip + count]
+    if not sel:
+        raise ValueError(f"no dispatches {skip}..{skip + count - 1} of {kernel} under {directory}")
+    us = [(int(r["end_timestamp"]) - int(r["start_timestamp"])) * 1e-3 for r in sel]
+    return sum(us) / len(us), len(us)
+
+
 def counter_per_dispatch(directory: str, counter: str, match: str, min_value: float = 0.0) -> tuple[float, int]:
     """Mean of ``counter`` over the dispatches of kernels whose name contains ``match`` (and whose value is at least
     ``min_value``); returns (mean, number of dispatches)."""
@@ -142,6 +154,9 @@ def main():
         fig["bench"] = {"value": bench["value"], "kernel_ms": rf["kernel_ms"],
                         "bytes_per_env_step": rf["bytes_per_env_step"], "n_envs": bench["config"]["n_envs"]}
         fig["algorithmic_bytes_per_launch"] = rf["bytes_per_env_step"] * bench["config"]["n_envs"]
+        # bench.py runs its warm-up substeps, then the timed ones: the same dispatches of the trace
+        fig["timed_avg_us"], fig["timed_dispatches"] = dispatch_avg_us(a.trace, a.kernel, int(bench["warmup"]),
+                                                                       int(bench["steps"]))
     with open(a.out + ".json", "w") as f:
         json.dump(fig, f, indent=1)
     lines = [a.title or f"{a.kernel}: rocprofv3 summary", "=" * 100, "", "Commands (repo root, one MI355X):", ""]
@@ -171,8 +186,10 @@ def main():
         lines += ["", f"Algorithmic bytes per launch: {rf['bytes_per_env_step']:.1f} B/env-step x "
                       f"{bench['config']['n_envs']} envs = {alg / 1e6:.3f} MB; traffic / algorithmic = "
                       f"{tr['bytes_per_launch'] / alg:.2f}",
-                  f"bench.py on the traced run: kernel_ms {rf['kernel_ms']:.4f} (HIP events on the launch stream) vs "
-                  f"rocprof average {k['avg_us'] / 1e3:.4f} ms"]
+                  f"bench.py on the traced run: kernel_ms {rf['kernel_ms']:.4f} (HIP events on the launch stream over "
+                  f"its timed region) vs rocprof {fig['timed_avg_us'] / 1e3:.4f} ms (mean of the same "
+                  f"{fig['timed_dispatches']} dispatches, after the {int(bench['warmup'])} warm-up ones, from the "
+                  f"per-dispatch trace) and {k['avg_us'] / 1e3:.4f} ms (all {k['calls']} dispatches, --stats)"]
     lines += ["", f"source digest (kernel sources + build flags): {fig['source_digest']}"]
     with open(a.out + ".md", "w") as f:
         f.write("\n".join(lines) + "\n")
