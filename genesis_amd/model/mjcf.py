@@ -78,9 +78,11 @@ class _Defaults:
 
 
 class MJCFLoader:
-    def __init__(self, path: str):
+    def __init__(self, path: str, root: ET.Element | None = None):
+        """``root``: an already-built ``<mujoco>`` tree (the URDF loader's translation); relative mesh files are then
+        resolved against ``path``'s directory as for a file."""
         self.path = path
-        self.root = ET.parse(path).getroot()
+        self.root = ET.parse(path).getroot() if root is None else root
         comp = self.root.find("compiler")
         self.angle_deg = comp is not None and comp.attrib.get("angle", "degree") == "degree"
         self.autolimits = comp is None or comp.attrib.get("autolimits", "true") == "true"
@@ -101,12 +103,15 @@ class MJCFLoader:
         else:
             self.classes["main"] = main
         self.mesh_files = {}
+        self.mesh_scale = {}
         asset = self.root.find("asset")
         if asset is not None:
             for m in asset.findall("mesh"):
                 f = m.attrib["file"]
                 name = m.attrib.get("name", os.path.splitext(os.path.basename(f))[0])
                 self.mesh_files[name] = os.path.join(self.meshdir, f)
+                if "scale" in m.attrib:
+                    self.mesh_scale[name] = _floats(m.attrib["scale"], 3)
         self._mesh_cache = {}
 
     def _collect(self, elem, parent):
@@ -142,6 +147,8 @@ class MJCFLoader:
     def _mesh(self, name):
         if name not in self._mesh_cache:
             verts, faces = mu.load_mesh(self.mesh_files[name])
+            if name in self.mesh_scale:
+                verts = verts * self.mesh_scale[name]
             # MuJoCo stores mesh vertices in the mesh's inertial frame; the geom pose absorbs that frame.
             vol, com, inertia = mu.mass_properties(verts, faces)
             if vol <= 0.0:

@@ -1,7 +1,10 @@
 """Morph descriptors (``gs.morphs``) for the shapes the rigid step supports.
 
-Mirrors ``genesis/options/morphs.py``: ``Plane`` (``:427-468``), ``Box``, ``Sphere``, ``MJCF`` (file path relative to
-the asset directory or absolute). Each morph turns into an entity spec for the model compiler.
+Mirrors ``genesis/options/morphs.py``: ``Plane`` (``:427-468``), ``Box``, ``Sphere``, ``MJCF`` and ``URDF``
+(``:916-1030``; file paths relative to an asset directory or absolute). Each morph turns into an entity spec for the
+model compiler. Asset directories: the package's own ``assets`` and, when set, the directories listed in the
+``GENESIS_AMD_ASSET_DIR`` environment variable (``os.pathsep``-separated), e.g. a Genesis checkout's
+``genesis/assets``.
 """
 
 from __future__ import annotations
@@ -12,22 +15,25 @@ import numpy as np
 
 from .model import primitives
 from .model.mjcf import load_mjcf
+from .model.urdf import DEFAULT_ARMATURE, load_urdf
 from .model import transforms as tf
 
-ASSET_DIRS = [
-    os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets"),
-    "/root/reference/genesis/assets",
-]
+ASSET_DIRS = [os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets")]
+
+
+def asset_dirs() -> list[str]:
+    extra = [d for d in os.environ.get("GENESIS_AMD_ASSET_DIR", "").split(os.pathsep) if d]
+    return ASSET_DIRS + extra
 
 
 def find_asset(path: str) -> str:
     if os.path.isabs(path) and os.path.exists(path):
         return path
-    for d in ASSET_DIRS + [os.getcwd()]:
+    for d in asset_dirs() + [os.getcwd()]:
         p = os.path.join(d, path)
         if os.path.exists(p):
             return p
-    raise FileNotFoundError(f"asset '{path}' not found in {ASSET_DIRS}")
+    raise FileNotFoundError(f"asset '{path}' not found in {asset_dirs()}")
 
 
 class Morph:
@@ -105,8 +111,28 @@ class MJCF(Morph):
 
 
 class URDF(Morph):
-    def __init__(self, file: str, **kwargs):
-        raise NotImplementedError("URDF loading is outside the rigid-step build (MJCF, Plane, Box, Sphere only)")
+    """``options/morphs.py:916-1030``: ``fixed`` (default False: the base link floats on a free joint),
+    ``merge_fixed_links`` / ``links_to_keep``, ``default_armature``; ``pos``/``quat``/``euler`` place the base."""
+
+    def __init__(self, file: str, scale: float = 1.0, merge_fixed_links: bool = True, links_to_keep=(),
+                 default_armature: float | None = DEFAULT_ARMATURE, fixed: bool = False, **kwargs):
+        super().__init__(fixed=fixed, **kwargs)
+        if scale != 1.0:
+            raise NotImplementedError("URDF scale != 1 is not supported")
+        self.file = file
+        self.merge_fixed_links = merge_fixed_links
+        self.links_to_keep = tuple(links_to_keep)
+        self.default_armature = default_armature
+
+    def to_spec(self, material, name):
+        spec = load_urdf(find_asset(self.file), name, fixed=self.fixed, merge_fixed=self.merge_fixed_links,
+                         links_to_keep=self.links_to_keep, default_armature=self.default_armature, pos=self.pos,
+                         quat=self.quat)
+        if material is not None and material.friction is not None:
+            for link in spec.links:
+                for g in link.geoms:
+                    g.friction = material.friction
+        return spec
 
 
 SPEC_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "assets", "specs")
