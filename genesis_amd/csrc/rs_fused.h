@@ -323,12 +323,16 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     const int lane = lane_id();
     float* wk = st.wk + (size_t)b * fa.work_stride;
     float* slice = lds_env + (size_t)w * fa.lds_wave;
-    float* base = kLds ? slice : wk + fa.off_loc;
+    // With one record per environment (st.LS > 0) the record holds the local rows word for word as the private region
+    // lays them out, so a launch that keeps the private region in global memory works on the record in place instead
+    // of copying it into the work field and back (for 120 dofs that copy is three 120 x 120 matrices each way).
+    const bool in_place = !kLds && st.LS > 0;
+    float* base = kLds ? slice : in_place ? (float*)((uint32_t*)(void*)st.qpos + (size_t)b * st.LS) : wk + fa.off_loc;
     float* tile = slice + fa.tile_off;
     const LState loc = make_local(base, fa);
     const WorkRows R{wk + fa.off_jac, wk + fa.off_rowf, wk + fa.off_tmp, m.CMAX, m.D};
 
-    stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
+    if (!in_place) stage_local<true>(st, (uint32_t*)base, fa.n_local, b, lane);
     wave_sync();
     // An environment that had many constraint rows in its last substep is likely the longest of this one: it takes
     // issue priority over the other wavefronts of its SIMD, which shortens the step's tail (row-tile models, whose
@@ -369,7 +373,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     dev_integrate(m, loc, 0);
     wave_sync();
     wave_kinematics(m, loc, lane);
-    stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
+    if (!in_place) stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
 }
 
 // Consecutive environments go to one XCD (workgroups are dealt round-robin over the 8 XCDs), so that the planes'
@@ -421,7 +425,7 @@ k_fused_wave(DModel m, DState st, FusedArgs fa) {
     if (b >= st.B) return;  // whole wavefronts only
     fused_env<kLds, kPath>(m, st, fa, b, w, lds_env);
     if (fa.sched && lane_id() == 0) {
-        const int cls = sched_class(st.n_con[b]);
+        const int cls = sched_class(FV(st, n_con, b)[0]);
         const int q = par ^ 1;
         const int slot = atomicAdd(&ctl[1 + q * kSchedClasses + cls], 1);
         if (slot < st.B) st.sched[(size_t)(q * kSchedClasses + cls) * st.B + slot] = b;

@@ -520,15 +520,17 @@ __device__ inline void wave_gradient_cg(const DModel& m, const LState& loc) {
 }
 
 // solver.py:2722-2751 + :1893-1910 for models with 25 to 128 dofs (whose factor comes from the matrix cores),
-// lane-parallel: lane l holds rows l and l + 64. Forward substitution column by column (row i subtracts L[i][j] x_j for
-// j = 0, 1, ... in turn, the reference's order for row i); backward substitution row by row, row i's sum of
-// L[j][i] x_j over j > i formed by lanes j and reduced across the wavefront. The factor may live in global memory
-// (large models keep their private region there), so its columns are loaded kAhead steps early.
+// lane-parallel with lane l holding entries l and l + 64, reading the factor by rows only (a row of the row-major
+// lower triangle is contiguous, so each load is coalesced; a column would be 64 scattered rows). Forward substitution
+// row by row: y_i = (g_i - sum over k < i of L[i][k] y_k) / L[i][i], the sum formed by lanes k and reduced across the
+// wavefront. Backward substitution column by column: x_i = y_i / L[i][i], then every lane k < i subtracts L[i][k] x_i
+// from its y_k (row i again). The sums run in their own order (a tree forward, descending i backward), as the
+// matrix-core factor already does. Rows are loaded kAhead steps early (the factor may live in global memory).
 __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, int lane) {
     constexpr int kAhead = 4;
     const int D = m.D;
     const float* L = FP(loc, nt_H);
-    float cur[2], hd[2], x[2] = {0.f, 0.f};
+    float cur[2], hd[2], y[2] = {0.f, 0.f}, x[2] = {0.f, 0.f};
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         const int i = lane + r * kWave;
@@ -541,56 +543,51 @@ __device__ inline void wave_gradient_lanes(const DModel& m, const LState& loc, i
             hd[r] = L[i * D + i];
         }
     }
-    // forward: pf[k][r] = L[row r of this lane][column j0 + k]
-    auto colv = [&](int i, int j) { return (j >= 0 && i < D && i > j) ? L[i * D + j] : 0.f; };
+    // L[i][k] for k < i (row i, strictly below the diagonal), else 0
+    auto rowv = [&](int i, int k) { return (i >= 0 && i < D && k < i) ? L[i * D + k] : 0.f; };
     float pf[kAhead][2];
 #pragma unroll
     for (int k = 0; k < kAhead; k++)
 #pragma unroll
-        for (int r = 0; r < 2; r++) pf[k][r] = colv(lane + r * kWave, k);
-    for (int j0 = 0; j0 < D; j0 += kAhead) {
+        for (int r = 0; r < 2; r++) pf[k][r] = rowv(k, lane + r * kWave);
+    for (int i0 = 0; i0 < D; i0 += kAhead) {
 #pragma unroll
         for (int k = 0; k < kAhead; k++) {
-            const int j = j0 + k;
-            if (j < D) {
-                const int rj = j >> 6, lj = j & (kWave - 1);
-                const float xj = readlane_f(rj ? cur[1] : cur[0], lj) / readlane_f(rj ? hd[1] : hd[0], lj);
-                if (lane == lj) {
-                    if (rj) x[1] = xj;
-                    else x[0] = xj;
-                }
+            const int i = i0 + k;
+            if (i < D) {
+                const float prod = pf[k][0] * y[0] + pf[k][1] * y[1];  // y_k is zero until set, L[i][k] zero for k >= i
 #pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    const int i = lane + r * kWave;
-                    if (i < D && i > j) cur[r] = cur[r] - pf[k][r] * xj;
-                    pf[k][r] = colv(i, j + kAhead);
+                for (int r = 0; r < 2; r++) pf[k][r] = rowv(i + kAhead, lane + r * kWave);
+                const int ri = i >> 6, li = i & (kWave - 1);
+                float c = readlane_f(ri ? cur[1] : cur[0], li) - wave_sum_f(prod);
+                c = c / readlane_f(ri ? hd[1] : hd[0], li);
+                if (lane == li) {
+                    if (ri) y[1] = c;
+                    else y[0] = c;
                 }
             }
         }
     }
-    // backward: pb[k][r] = L[row r of this lane][column i], for i = D - 1 - (ii0 + k). Row i's sum over j > i of
-    // L[j][i] x_j is a wavefront reduction (a tree, not the reference's order j = i + 1, i + 2, ...: these models'
-    // factor comes from the matrix cores anyway), so each row costs a reduction instead of a chain of D - i readlanes.
     float pb[kAhead][2];
 #pragma unroll
     for (int k = 0; k < kAhead; k++)
 #pragma unroll
-        for (int r = 0; r < 2; r++) pb[k][r] = colv(lane + r * kWave, D - 1 - k);
+        for (int r = 0; r < 2; r++) pb[k][r] = rowv(D - 1 - k, lane + r * kWave);
     for (int ii0 = 0; ii0 < D; ii0 += kAhead) {
 #pragma unroll
         for (int k = 0; k < kAhead; k++) {
             const int i = D - 1 - (ii0 + k);
             if (i >= 0) {
-                // lane j: L[j][i] x_j (zero unless j > i)
-                const float prod = pb[k][0] * x[0] + pb[k][1] * x[1];
-#pragma unroll
-                for (int r = 0; r < 2; r++) pb[k][r] = colv(lane + r * kWave, i - kAhead);
                 const int ri = i >> 6, li = i & (kWave - 1);
-                float c = readlane_f(ri ? x[1] : x[0], li) - wave_sum_f(prod);
-                c = c / readlane_f(ri ? hd[1] : hd[0], li);
+                const float xi = readlane_f(ri ? y[1] : y[0], li) / readlane_f(ri ? hd[1] : hd[0], li);
                 if (lane == li) {
-                    if (ri) x[1] = c;
-                    else x[0] = c;
+                    if (ri) x[1] = xi;
+                    else x[0] = xi;
+                }
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    y[r] = y[r] - pb[k][r] * xi;  // L[i][k] is zero for lanes k >= i
+                    pb[k][r] = rowv(i - kAhead, lane + r * kWave);
                 }
             }
         }
