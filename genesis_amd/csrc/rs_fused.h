@@ -168,9 +168,11 @@ __device__ inline void wave_broadphase(const DModel& m, const LState& loc, const
 // detection) item); the pair's lane then merges their results in detection order, as the sequential loop does.
 // mc: per-pair records (kMcRec floats per lane), the item map and the item results, in the work field. Detections
 // handed to GJK/EPA run on the whole wavefront when there is one (polytope in gjk_lds, LDS or nullptr; in gjk_mem when
-// it outgrows that) and one per lane when there are several (polytopes in the lane slots at gjk_mem).
-__device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
-                                        float* mc, float* gjk_lds, float* gjk_mem, int lane) {
+// it outgrows that) and one per lane when there are several (polytopes in the lane slots at gjk_mem). Returns the
+// GJK/EPA work it ran (uniform): lane passes << 16 | single wavefront detections, the scheduler's cost hint.
+__device__ __forceinline__ int wave_narrowphase(const DModel& m, const LState& loc, const DState& st, int b, float* pc,
+                                       float* mc, float* gjk_lds, float* gjk_mem, int lane) {
+    int gjk_passes = 0, gjk_singles = 0;
     const int nb = FP(loc, n_broad)[0];
     int placed = 0;  // uniform: contacts produced so far
     SlotSink sink{pc + (size_t)lane * m.NCPP * 8, m.NCPP, 0};
@@ -242,6 +244,7 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
                         // field; the lane that owns a detection takes its result
                         uint64_t gm = wave_ballot(act && want_gjk);
                         if (gm && !(gm & (gm - 1))) {
+                            gjk_singles++;
                             const int l = __builtin_ctzll(gm);
                             const GjkPair gp = make_gjk_pair(m, rl_i(dga, l), rl_i(dgb, l), rl3(pa, l), rl4(qa, l),
                                                              rl3(pb, l), rl4(qb, l));
@@ -250,6 +253,7 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
                             const bool gcol = gjk_contact_wave(m, gp, lane, gjk_lds, gjk_mem, gn, gpos, gpen);
                             if (lane == l) col = apply_gjk(gcol, gn, gpos, gpen, n, cp, pen);
                         } else if (gm) {
+                            gjk_passes++;
                             const bool mine = (gm >> lane) & 1;
                             const GjkPair gp = make_gjk_pair(m, dga, dgb, pa, qa, pb, qb);
                             f3 gn = zero3(), gpos = zero3();
@@ -312,13 +316,15 @@ __device__ __forceinline__ void wave_narrowphase(const DModel& m, const LState& 
     if (placed > m.KMAX) FP(loc, errbits)[0] = FP(loc, errbits)[0] | RS_ERR_OVERFLOW_COLLISION_PAIRS;
     FP(loc, n_contacts)[0] = min(placed, m.KMAX);
     wave_sync();
+    return min(gjk_passes, 0x7fff) << 16 | min(gjk_singles, 0xffff);
 }
 
 // One environment's substep on the calling wavefront (w = wavefront index in the workgroup).
 // kPath: 1 = models in the first row-tile class (at most kTileMaxDofs dofs), 4 = the second (up to kTileWideDofs),
-// 0 = the others, 2 = any (decided per call: the shared launch of several models).
+// 0 = the others, 2 = any (decided per call: the shared launch of several models). Returns the narrowphase's GJK/EPA
+// work (see wave_narrowphase).
 template <bool kLds, int kPath = 2>
-__device__ __forceinline__ void fused_env(const DModel& m, const DState& st, const FusedArgs& fa, int b, int w,
+__device__ __forceinline__ int fused_env(const DModel& m, const DState& st, const FusedArgs& fa, int b, int w,
                                           float* lds_env) {
     const int lane = lane_id();
     float* wk = st.wk + (size_t)b * fa.work_stride;
@@ -346,6 +352,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     wave_update_acc(m, loc, lane, false);
     wave_rne_and_qacc(m, loc, lane);
     // Collider.detection
+    int gjk_work = 0;
     if (m.enable_collision) {
         for (int g = lane; g < m.G; g += kWave) dev_aabb(m, loc, 0, g);
         wave_sync();
@@ -353,8 +360,8 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
             const int n_old = FP(loc, n_contacts)[0];
             for (int i = lane; i < n_old; i += kWave) clear_contacts(st, b, i, i + 1);
             wave_broadphase(m, loc, st, b, lane);
-            wave_narrowphase(m, loc, st, b, wk + fa.off_pc, wk + fa.off_mc, fa.gjk_off >= 0 ? slice + fa.gjk_off : nullptr,
-                             wk + fa.off_gjk, lane);
+            gjk_work = wave_narrowphase(m, loc, st, b, wk + fa.off_pc, wk + fa.off_mc,
+                                        fa.gjk_off >= 0 ? slice + fa.gjk_off : nullptr, wk + fa.off_gjk, lane);
         }
     }
     // ConstraintSolver.resolve
@@ -374,6 +381,7 @@ __device__ __forceinline__ void fused_env(const DModel& m, const DState& st, con
     wave_sync();
     wave_kinematics(m, loc, lane);
     if (!in_place) stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
+    return gjk_work;
 }
 
 // Consecutive environments go to one XCD (workgroups are dealt round-robin over the 8 XCDs), so that the planes'
@@ -423,9 +431,9 @@ k_fused_wave(DModel m, DState st, FusedArgs fa) {
         }
     }
     if (b >= st.B) return;  // whole wavefronts only
-    fused_env<kLds, kPath>(m, st, fa, b, w, lds_env);
+    const int gjk_work = fused_env<kLds, kPath>(m, st, fa, b, w, lds_env);
     if (fa.sched && lane_id() == 0) {
-        const int cls = sched_class(FV(st, n_con, b)[0]);
+        const int cls = sched_class(FV(st, n_con, b)[0], gjk_work);
         const int q = par ^ 1;
         const int slot = atomicAdd(&ctl[1 + q * kSchedClasses + cls], 1);
         if (slot < st.B) st.sched[(size_t)(q * kSchedClasses + cls) * st.B + slot] = b;

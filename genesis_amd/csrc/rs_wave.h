@@ -57,8 +57,15 @@ __host__ __device__ constexpr int tile_class(int n_dofs) {
 constexpr int kSchedClasses = 4;
 __host__ __device__ constexpr int sched_rows() { return 2 * kSchedClasses + 1; }
 constexpr int kSchedHeavyRows = 150;
-__device__ __forceinline__ int sched_class(int n_con) {
-    return n_con >= kSchedHeavyRows ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
+// gjk_work: the GJK/EPA work of the environment's last narrowphase, lane passes << 16 | single wavefront detections.
+// A lane pass (up to 64 detections, one per lane) outweighs any number of rows, so it sets the class when present.
+__device__ __forceinline__ int sched_class(int n_con, int gjk_work = 0) {
+    const int by_rows = n_con >= kSchedHeavyRows ? 0 : n_con >= 60 ? 1 : n_con >= 20 ? 2 : 3;
+    const int passes = gjk_work >> 16, singles = gjk_work & 0xffff;
+    if (passes >= 2) return 0;
+    if (passes == 1) return min(by_rows, 1);
+    if (singles > 0) return min(by_rows, 2);
+    return by_rows;
 }
 
 // perturbed multi-contact detections of the fused narrowphase: per-pair record and per-detection result (floats)
