@@ -8,6 +8,7 @@
 * ``traffic(...)``: HBM bytes per launch of a kernel from a FETCH_SIZE run and a WRITE_SIZE run, each scaled by what
   the same counter reads for a streaming copy of known size in the same run (the calibration kernel);
 * ``valu(...)``: VALU busy share and lane utilisation of a kernel from one ``--pmc`` run of VALU_COUNTERS;
+* ``mfma(...)``: matrix-core FP32 operations and busy share of a kernel from one ``--pmc`` run of MFMA_COUNTERS;
 * ``source_digest()``: digest of the kernel sources and build flags, recorded with a profile so that a report can tell
   whether the profile belongs to the kernels it is reporting on.
 
@@ -75,28 +76,31 @@ def dispatch_avg_us(directory: str, kernel: str, skip: int, count: int) -> tuple
     return sum(us) / len(us), len(us)
 
 
-def counter_per_dispatch(directory: str, counter: str, match: str, min_value: float = 0.0) -> tuple[float, int]:
+def counter_per_dispatch(directory: str, counter: str, match: str, min_value: float = 0.0,
+                         skip: int = 0) -> tuple[float, int]:
     """Mean of ``counter`` over the dispatches of kernels whose name contains ``match`` (and whose value is at least
-    ``min_value``); returns (mean, number of dispatches)."""
+    ``min_value``), leaving out the first ``skip`` of them in dispatch order (a run's warm-up); returns (mean, number
+    of dispatches)."""
     vals: dict[str, float] = {}
     for r in _rows(directory, "counter_collection.csv"):
         if r.get("counter_name") != counter or match not in r.get("kernel_name", ""):
             continue
         key = r.get("dispatch_id") or r.get("correlation_id")
         vals[key] = vals.get(key, 0.0) + float(r["counter_value"])
-    sel = [v for v in vals.values() if v >= min_value]
+    ordered = [vals[k] for k in sorted(vals, key=lambda k: int(float(k)))][skip:]
+    sel = [v for v in ordered if v >= min_value]
     if not sel:
         raise ValueError(f"no dispatch of *{match}* with {counter}")
     return sum(sel) / len(sel), len(sel)
 
 
-def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: int) -> dict:
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: int, skip: int = 0) -> dict:
     """HBM bytes per launch of ``kernel``. FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by
     copy_bytes / (what it reads for the calibration copy, which reads and writes copy_bytes)."""
     out = {"kernel": kernel, "copy_bytes": copy_bytes}
     total = 0.0
     for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
-        k_kib, n_k = counter_per_dispatch(d, name, kernel)
+        k_kib, n_k = counter_per_dispatch(d, name, kernel, skip=skip)
         c_kib, n_c = counter_per_dispatch(d, name, CALIBRATION_MATCH, min_value=0.5 * copy_bytes / 1024)
         factor = copy_bytes / (c_kib * 1024.0)
         out[name] = {"kernel_kib_per_dispatch": k_kib, "kernel_dispatches": n_k, "copy_kib_per_dispatch": c_kib,
@@ -109,7 +113,7 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: int) -> dic
 VALU_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "GRBM_GUI_ACTIVE")
 
 
-def valu(directory: str, kernel: str, simds: int, xcds: int = 8) -> dict:
+def valu(directory: str, kernel: str, simds: int, xcds: int = 8, skip: int = 0) -> dict:
     """VALU figures of ``kernel`` from one ``--pmc`` run of VALU_COUNTERS (means per dispatch). GRBM_GUI_ACTIVE is
     summed over the device's XCDs (it reads about xcds times the kernel's duration in cycles), so the device's busy
     cycles are GRBM_GUI_ACTIVE / xcds:
@@ -117,9 +121,26 @@ def valu(directory: str, kernel: str, simds: int, xcds: int = 8) -> dict:
     valu_lane_util = SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU), the share of lanes active in them."""
     out = {"simds": simds, "xcds": xcds}
     for c in VALU_COUNTERS:
-        out[c], out["dispatches"] = counter_per_dispatch(directory, c, kernel)
+        out[c], out["dispatches"] = counter_per_dispatch(directory, c, kernel, skip=skip)
     out["valu_busy"] = out["SQ_ACTIVE_INST_VALU"] * 4.0 / (simds * out["GRBM_GUI_ACTIVE"] / xcds)
     out["valu_lane_util"] = out["SQ_THREAD_CYCLES_VALU"] / (64.0 * out["SQ_ACTIVE_INST_VALU"])
+    return out
+
+
+MFMA_COUNTERS = ("SQ_INSTS_VALU_MFMA_MOPS_F32", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_INSTS_MFMA", "GRBM_GUI_ACTIVE")
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+
+
+def mfma(directory: str, kernel: str, simds: int, xcds: int = 8, skip: int = 0) -> dict:
+    """Matrix-core figures of ``kernel`` from one ``--pmc`` run of MFMA_COUNTERS (means per dispatch):
+    flops = SQ_INSTS_VALU_MFMA_MOPS_F32 * 512 (the counter counts FP32 matrix multiply-adds in units of 512 FLOPs);
+    mfma_busy = SQ_VALU_MFMA_BUSY_CYCLES / (simds * GRBM_GUI_ACTIVE / xcds), the share of SIMD cycles the matrix
+    cores were busy (GRBM_GUI_ACTIVE summed over the XCDs, as in ``valu``)."""
+    out = {"simds": simds, "xcds": xcds}
+    for c in MFMA_COUNTERS:
+        out[c], out["dispatches"] = counter_per_dispatch(directory, c, kernel, skip=skip)
+    out["flops"] = out["SQ_INSTS_VALU_MFMA_MOPS_F32"] * 512.0
+    out["mfma_busy"] = out["SQ_VALU_MFMA_BUSY_CYCLES"] / (simds * out["GRBM_GUI_ACTIVE"] / xcds)
     return out
 
 
@@ -134,20 +155,24 @@ def main():
     ap.add_argument("--title", default="")
     ap.add_argument("--commands", default="", help="text of the commands that produced the runs")
     ap.add_argument("--valu", default=None, help="directory of a --pmc run of VALU_COUNTERS")
+    ap.add_argument("--mfma", default=None, help="directory of a --pmc run of MFMA_COUNTERS")
+    ap.add_argument("--pmc-skip", type=int, default=0,
+                    help="leave out each counter run's first N dispatches of the kernel (its warm-up substeps)")
     ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the device (MI355X: 256 CUs x 4)")
     ap.add_argument("--xcds", type=int, default=8, help="XCDs of the device (MI355X: 8)")
     ap.add_argument("--out", required=True, help="output path without extension")
     a = ap.parse_args()
     stats = kernel_stats(a.trace)
-    tr = traffic(a.fetch, a.write, a.kernel, a.copy_bytes)
-    vu = valu(a.valu, a.kernel, a.simds, a.xcds) if a.valu else None
+    tr = traffic(a.fetch, a.write, a.kernel, a.copy_bytes, a.pmc_skip)
+    vu = valu(a.valu, a.kernel, a.simds, a.xcds, a.pmc_skip) if a.valu else None
+    mf = mfma(a.mfma, a.kernel, a.simds, a.xcds, a.pmc_skip) if a.mfma else None
     k = next(s for s in stats if a.kernel in s["name"])
     bench = None
     if a.bench_json:
         with open(a.bench_json) as f:
             lines = [ln for ln in f if ln.startswith("{")]
         bench = json.loads(lines[-1]) if lines else None
-    fig = {"kernel": a.kernel, "avg_us": k["avg_us"], "calls": k["calls"], "traffic": tr, "valu": vu,
+    fig = {"kernel": a.kernel, "avg_us": k["avg_us"], "calls": k["calls"], "traffic": tr, "valu": vu, "mfma": mf,
            "source_digest": source_digest(), "title": a.title}
     if bench:
         rf = bench["roofline"]
@@ -166,7 +191,8 @@ def main():
     for s in stats[:12]:
         nm = s["name"] if len(s["name"]) <= 60 else s["name"][:57] + "..."
         lines.append(f"{nm:60s} {s['calls']:7d} {s['avg_us']:10.1f} {s['total_ms']:10.2f} {s['share']:6.1f}%")
-    lines += ["", f"HBM traffic of {a.kernel} (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB per dispatch):", ""]
+    skipped = f", after the first {a.pmc_skip} (warm-up) dispatches" if a.pmc_skip else ""
+    lines += ["", f"HBM traffic of {a.kernel} (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB per dispatch{skipped}):", ""]
     for name in ("FETCH_SIZE", "WRITE_SIZE"):
         t = tr[name]
         lines.append(f"  {name}: kernel {t['kernel_kib_per_dispatch']:.1f} KiB/dispatch ({t['kernel_dispatches']} "
@@ -181,6 +207,16 @@ def main():
                   f"  VALU busy (SQ_ACTIVE_INST_VALU * 4 / (SIMDs * GRBM_GUI_ACTIVE / XCDs)): {100 * vu['valu_busy']:.1f}%",
                   f"  VALU lane utilisation (SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU)): "
                   f"{100 * vu['valu_lane_util']:.1f}%"]
+    if mf:
+        t_ms = fig.get("timed_avg_us", k["avg_us"]) / 1e3
+        lines += ["", f"Matrix cores of {a.kernel} (--pmc {' '.join(MFMA_COUNTERS)}; means per dispatch over "
+                      f"{mf['dispatches']} dispatches):",
+                  "", "  " + ", ".join(f"{c} {mf[c]:.4g}" for c in MFMA_COUNTERS),
+                  f"  FP32 matrix FLOPs per dispatch (SQ_INSTS_VALU_MFMA_MOPS_F32 * 512): {mf['flops']:.4g}; over the "
+                  f"kernel's mean duration (timed region) {t_ms:.4f} ms: {mf['flops'] / (t_ms * 1e-3) / 1e12:.3f} TFLOP/s of "
+                  f"{MFMA_FP32_PEAK_TFLOPS} ({100 * mf['flops'] / (t_ms * 1e-3) / 1e12 / MFMA_FP32_PEAK_TFLOPS:.3f}%)",
+                  f"  matrix cores busy (SQ_VALU_MFMA_BUSY_CYCLES / (SIMDs * GRBM_GUI_ACTIVE / XCDs)): "
+                  f"{100 * mf['mfma_busy']:.2f}%"]
     if bench:
         alg = fig["algorithmic_bytes_per_launch"]
         lines += ["", f"Algorithmic bytes per launch: {rf['bytes_per_env_step']:.1f} B/env-step x "
