@@ -368,7 +368,7 @@ def main():
     ap.add_argument("--no-drift", action="store_true", help="skip the FP32-vs-FP64 drift sample")
     ap.add_argument("--like-for-like", action=argparse.BooleanOptionalAction, default=True,
                     help="franka: also run the reference's speed benchmark (B=30000, fixed target) via the Scene API")
-    ap.add_argument("--fused-config", default=None, metavar="EPB,LDS,SCHED,OCC",
+    ap.add_argument("--fused-config", default=None, metavar="EPB,LDS,SCHED,OCC[,GJK_LDS]",
                     help="fixed fused launch shape (e.g. 1,1,0,0) instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -394,8 +394,8 @@ def main():
           file=sys.stderr, flush=True)
     tune = None
     if args.fused_config:
-        epb, lds, sched, occ = (int(x) for x in args.fused_config.split(","))
-        dm.set_fused_config(epb, bool(lds), bool(sched), occ)
+        epb, lds, sched, occ, *gjk = (int(x) for x in args.fused_config.split(","))
+        dm.set_fused_config(epb, bool(lds), bool(sched), occ, bool(gjk[0]) if gjk else False)
     elif not args.no_autotune:
         probe = make_slab(wl, dm, B)
         probe.step(20, rt.RS_MODE_FUSED)  # tune on a state with contacts, not the initial one

@@ -66,6 +66,7 @@ struct rs_model {
     int layout = RS_LAYOUT_ENV_RECORDS;  // state layout of the per-environment (local-kind) fields
     int max_lds = 0;    // device limit of dynamic LDS per workgroup (bytes)
     int n_cu = 0;       // compute units of the device
+    bool has_mesh = false;  // convex mesh geoms: the GJK/EPA polytope may be kept in LDS (gjk_lds)
 };
 
 // Shared launches of several (model, slab) batches: a ring of pinned host + device segment tables, each reused only
@@ -466,11 +467,11 @@ static int fused_substep(const rs_model* m, const DState& st, hipStream_t s) {
     return RS_OK;
 }
 
-static int check_fused_config(const rs_model* m, int epb, int use_lds) {
+static int check_fused_config(const rs_model* m, const FusedArgs& fa, int epb, int use_lds) {
     if (epb < 1 || epb > 8) return fail(RS_E_INVALID, "fused config: envs_per_block must be in 1..8");
     if (!m->fa.tiled && epb * kWave > kGenericMaxThreads)
         return fail(RS_E_UNSUPPORTED, "fused config: models above 24 dofs run at most 4 environments per workgroup");
-    if ((int64_t)epb * lds_slice_floats(m->h, m->fa, use_lds) * 4 > m->max_lds)
+    if ((int64_t)epb * lds_slice_floats(m->h, fa, use_lds) * 4 > m->max_lds)
         return fail(RS_E_UNSUPPORTED, use_lds ? "fused config: private regions do not fit in LDS at this envs_per_block"
                                               : "fused config: row tiles do not fit in LDS at this envs_per_block");
     return RS_OK;
@@ -586,16 +587,18 @@ int rs_model_create(const rs_model_header* h, const void* blob, size_t blob_byte
     fused_layout(m->h, &m->fa);
     d.work_stride = m->fa.work_stride;
     d.off_gjk = m->fa.off_gjk;
-    // models with convex mesh geoms meet the GJK/EPA hand-over often (deep overlaps of hulls): their wavefronts keep
-    // a small polytope in LDS; the others run the rare detection with the polytope in the work field
-    m->fa.gjk_words = has_convex_mesh(*h, (const char*)blob) ? kGjkLdsWords : 0;
+    // models with convex mesh geoms meet the GJK/EPA hand-over often (deep overlaps of hulls): their wavefronts may
+    // keep a small polytope in LDS (gjk_lds); by default it stays in the work field, which leaves LDS for more
+    // wavefronts per compute unit (Franka: 11.5 instead of 17.9 KB per environment)
+    m->has_mesh = has_convex_mesh(*h, (const char*)blob);
+    m->fa.gjk_words = 0;
     int max_lds = 0;
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device) != hipSuccess) max_lds = 0;
     m->max_lds = max_lds < 65536 ? max_lds : 65536;
     if (hipDeviceGetAttribute(&m->n_cu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess) m->n_cu = 0;
     // default launch shape: two environments per workgroup, private regions in LDS when they fit
     m->fa.epb = 2;
-    m->use_lds = check_fused_config(m, 2, 1) == RS_OK;
+    m->use_lds = check_fused_config(m, m->fa, 2, 1) == RS_OK;
     g_err.clear();
     *out = m;
     return RS_OK;
@@ -619,12 +622,17 @@ int rs_fused_config_get(const rs_model* m, rs_fused_config* out) {
     out->lds_bytes_per_env = lds_slice_floats(m->h, m->fa, m->use_lds) * 4;
     out->schedule = m->schedule;
     out->occupancy = m->occupancy;
+    out->gjk_lds = m->fa.gjk_words > 0;
     return RS_OK;
 }
 
 int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     if (!m || !c) return fail(RS_E_INVALID, "rs_fused_config_set: null argument");
-    if (int r = check_fused_config(m, c->envs_per_block, c->use_lds != 0)) return r;
+    if (c->gjk_lds && !m->has_mesh)
+        return fail(RS_E_UNSUPPORTED, "fused config: gjk_lds applies to models with convex mesh geoms only");
+    FusedArgs fa = m->fa;
+    fa.gjk_words = c->gjk_lds ? kGjkLdsWords : 0;
+    if (int r = check_fused_config(m, fa, c->envs_per_block, c->use_lds != 0)) return r;
     if (!m->fa.tiled && c->occupancy != 0)
         return fail(RS_E_UNSUPPORTED, "fused config: occupancy variants exist for models with a row tile only");
     if (m->fa.tiled == kTileWideDofs && c->occupancy == 4)
@@ -633,6 +641,7 @@ int rs_fused_config_set(rs_model* m, const rs_fused_config* c) {
     m->use_lds = c->use_lds != 0;
     m->schedule = c->schedule != 0;
     m->occupancy = (c->occupancy == 3 || c->occupancy == 4) ? c->occupancy : 0;
+    m->fa.gjk_words = fa.gjk_words;
     return RS_OK;
 }
 

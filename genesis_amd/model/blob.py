@@ -13,7 +13,7 @@ import numpy as np
 
 from .compiler import CompiledModel
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_MODEL_ARRAYS = 96
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _INCLUDE = os.path.normpath(os.path.join(_HERE, "..", "..", "include"))

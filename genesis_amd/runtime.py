@@ -57,7 +57,8 @@ class _ArrayInfo(ctypes.Structure):
 
 class FusedConfig(ctypes.Structure):
     _fields_ = [("envs_per_block", ctypes.c_int32), ("use_lds", ctypes.c_int32),
-                ("lds_bytes_per_env", ctypes.c_int32), ("schedule", ctypes.c_int32), ("occupancy", ctypes.c_int32)]
+                ("lds_bytes_per_env", ctypes.c_int32), ("schedule", ctypes.c_int32), ("occupancy", ctypes.c_int32),
+                ("gjk_lds", ctypes.c_int32)]
 
 
 _lib = None
@@ -123,7 +124,7 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.rs_abi_version() != 2:
+    if lib.rs_abi_version() != 3:
         raise RigidStepError("librigidstep ABI version mismatch")
     _lib = lib
     return lib
@@ -219,10 +220,12 @@ class DeviceModel:
         c = FusedConfig()
         _check(_lib.rs_fused_config_get(self.handle, ctypes.byref(c)))
         return {"envs_per_block": c.envs_per_block, "use_lds": bool(c.use_lds),
-                "lds_bytes_per_env": c.lds_bytes_per_env, "schedule": bool(c.schedule), "occupancy": c.occupancy}
+                "lds_bytes_per_env": c.lds_bytes_per_env, "schedule": bool(c.schedule), "occupancy": c.occupancy,
+                "gjk_lds": bool(c.gjk_lds)}
 
-    def set_fused_config(self, envs_per_block: int, use_lds: bool, schedule: bool = True, occupancy: int = 0):
-        c = FusedConfig(envs_per_block, int(use_lds), 0, int(schedule), int(occupancy))
+    def set_fused_config(self, envs_per_block: int, use_lds: bool, schedule: bool = True, occupancy: int = 0,
+                         gjk_lds: bool = False):
+        c = FusedConfig(envs_per_block, int(use_lds), 0, int(schedule), int(occupancy), int(gjk_lds))
         _check(_lib.rs_fused_config_set(self.handle, ctypes.byref(c)))
 
 
@@ -377,7 +380,8 @@ def autotune_fused(slab: StateSlab, steps: int = 10, candidates=None, stream=Non
         timings[cand] += [time_one(cand) for _ in range(repeats)]
     del scratch
     if not timings:
-        dm.set_fused_config(saved["envs_per_block"], saved["use_lds"], saved["schedule"], saved["occupancy"])
+        dm.set_fused_config(saved["envs_per_block"], saved["use_lds"], saved["schedule"], saved["occupancy"],
+                            saved.get("gjk_lds", False))
         raise RigidStepError("autotune: no runnable fused configuration")
     med = {c: float(np.median(t)) for c, t in timings.items()}
     best = min((c for c in timings if len(timings[c]) > 1), key=med.get, default=min(med, key=med.get))

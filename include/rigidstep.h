@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define RS_ABI_VERSION 2
+#define RS_ABI_VERSION 3
 
 enum {
     RS_OK = 0,
@@ -221,6 +221,8 @@ typedef struct rs_fused_config {
     int32_t lds_bytes_per_env;  /* out: private-region size of one environment */
     int32_t schedule;           /* 1: start the environments that were costliest in the previous substep first */
     int32_t occupancy;          /* 0: compiler's register budget; 3 or 4: kernel built for that many wavefronts per SIMD */
+    int32_t gjk_lds;            /* 1: GJK/EPA polytope of single detections in LDS (models with convex mesh geoms;
+                                   default 0: in the work field, leaving LDS for more wavefronts per compute unit) */
 } rs_fused_config;
 int rs_fused_config_get(const rs_model* model, rs_fused_config* out);
 int rs_fused_config_set(rs_model* model, const rs_fused_config* config);

@@ -101,19 +101,23 @@ def test_fused_launch_shapes_agree():
     want = _twin(rt, dm, ref)
     want.step(5, rt.RS_MODE_PLAIN)
     torch.cuda.synchronize()
-    shapes = [(epb, lds, sched, occ) for occ in (0, 3, 4) for sched in (True, False) for lds in (True, False)
+    # every shape with the GJK/EPA polytope of single detections in the work field (the default), and the LDS
+    # polytope (gjk_lds) on a few shapes
+    shapes = [(epb, lds, sched, occ, False) for occ in (0, 3, 4) for sched in (True, False) for lds in (True, False)
               for epb in (1, 2, 4, 8)]
-    for epb, lds, sched, occ in shapes:
+    shapes += [(1, True, True, 0, True), (2, True, False, 3, True), (4, False, True, 0, True)]
+    for epb, lds, sched, occ, gjk_lds in shapes:
         try:
-            dm.set_fused_config(epb, lds, sched, occ)
+            dm.set_fused_config(epb, lds, sched, occ, gjk_lds)
         except rt.RigidStepError:
             continue
-        print(f"[fused] launch shape epb={epb} lds={lds} schedule={sched} occupancy={occ}", flush=True)
+        print(f"[fused] launch shape epb={epb} lds={lds} schedule={sched} occupancy={occ} gjk_lds={gjk_lds}",
+              flush=True)
         s = _twin(rt, dm, ref)
         s.step(5, rt.RS_MODE_FUSED)
         torch.cuda.synchronize()
         for name in ("qpos", "vel", "acc", "n_contacts"):
-            assert _max_abs(get_dev(s, name), get_dev(want, name)) == 0.0, (epb, lds, sched, occ, name)
+            assert _max_abs(get_dev(s, name), get_dev(want, name)) == 0.0, (epb, lds, sched, occ, gjk_lds, name)
 
 
 @pytest.mark.parametrize("warm", [0, 40])
