@@ -101,6 +101,15 @@ __device__ __forceinline__ float wave_sum_f(float x) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), kWave - 1));
 }
 
+// sum over d < D of f(d) with lane l taking d = l, l + 64, ..., reduced across the wavefront (for the matrix-core
+// models, whose private region is in global memory: a loop over d on every lane would wait on one load per d)
+template <class F>
+__device__ __forceinline__ float wave_dsum(int D, int lane, F f) {
+    float p = 0.f;
+    for (int d = lane; d < D; d += kWave) p = p + f(d);
+    return wave_sum_f(p);
+}
+
 // out[d] = sum over rows c < nc of J[c][d] w[c] (d < D <= 128) on the matrix cores: for each 16-column tile, A holds
 // the row weights (A[i][k] = w[c0 + k] for every i) and B the rows (B[k][j] = J[c0 + k][16 T + j]), so every row of
 // the tile's result is the sum; 16 rows per trip, their operands loaded before the products.
@@ -208,12 +217,10 @@ __device__ inline float wave_update_constraint(const DModel& m, const LState& lo
             pq = pq + R.tmp(1, c);
         }
         float cost = wave_sum_f(pf);
-        float gauss = 0.f;
-        for (int d = 0; d < D; d++) {
-            float v = 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
-            gauss = gauss + v;
-            cost = cost + v;
-        }
+        const float gauss = wave_dsum(D, lane, [&](int d) {
+            return 0.5f * (LF(Ma, d) - LF(force, d)) * (LF(qacc, d) - LF(acc_smooth, d));
+        });
+        cost = cost + gauss;
         cost = cost + wave_sum_f(pq);
         LF(gauss, 0) = gauss;
         wave_sync();
@@ -942,10 +949,18 @@ __device__ inline LsP wave_ls_init_r(const DModel& m, const LState& loc, const W
     for (int d = 0; d < KR; d++) sr[d] = (KD > 0 && d < D) ? LF(search, d) : 0.f;
     wave_sync();
     float g1 = 0.f, g2 = 0.f;
-    for (int d = 0; d < D; d++) {
-        float s = LF(search, d);
-        g1 = g1 + (s * LF(Ma, d) - s * LF(force, d));
-        g2 = g2 + 0.5f * s * LF(mv, d);
+    if (KD == 0 && mfma_model(D)) {
+        g1 = wave_dsum(D, lane, [&](int d) {
+            const float s = LF(search, d);
+            return s * LF(Ma, d) - s * LF(force, d);
+        });
+        g2 = wave_dsum(D, lane, [&](int d) { return 0.5f * LF(search, d) * LF(mv, d); });
+    } else {
+        for (int d = 0; d < D; d++) {
+            float s = LF(search, d);
+            g1 = g1 + (s * LF(Ma, d) - s * LF(force, d));
+            g2 = g2 + 0.5f * s * LF(mv, d);
+        }
     }
     const float gauss = LF(gauss, 0);
     LF(quad_gauss, 0) = gauss;
@@ -1117,7 +1132,11 @@ template <int KD>
 __device__ inline float wave_linesearch(const DModel& m, const LState& loc, const WorkRows& R, int lane, float* tile) {
     const int D = m.D;
     float sn = 0.f;
-    for (int d = 0; d < D; d++) sn = sn + LF(search, d) * LF(search, d);
+    if (KD == 0 && mfma_model(D)) {
+        sn = wave_dsum(D, lane, [&](int d) { return LF(search, d) * LF(search, d); });
+    } else {
+        for (int d = 0; d < D; d++) sn = sn + LF(search, d) * LF(search, d);
+    }
     sn = sqrtf(sn);
     float gtol = m.tolerance * m.ls_tolerance * sn * (m.meaninertia * (float)max(1, D));
     LF(gtol, 0) = gtol;
@@ -1249,7 +1268,14 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
         else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
         else wave_gradient(m, loc);
     }
-    for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+    // models on the matrix-core path update their dof vectors lane-parallel (entries lane, lane + 64)
+    const bool lane_dofs = KD == 0 && mfma_model(D);
+    if (lane_dofs) {
+        for (int d = lane; d < D; d += kWave) LF(search, d) = -LF(Mgrad, d);
+        wave_sync();
+    } else {
+        for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+    }
     int n_iter = 0;
     if (nc > 0) {
         const float tol = (m.meaninertia * (float)max(1, D)) * m.tolerance;
@@ -1259,7 +1285,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
             if (fabsf(alpha) < m.eps) {
                 improved = 0;
             } else {
-                for (int d = 0; d < D; d++) {
+                for (int d = lane_dofs ? lane : 0; d < D; d += lane_dofs ? kWave : 1) {
                     LF(qacc, d) = LF(qacc, d) + LF(search, d) * alpha;
                     LF(Ma, d) = LF(Ma, d) + LF(mv, d) * alpha;
                 }
@@ -1289,11 +1315,20 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
                     else wave_gradient(m, loc);
                 }
                 float gn = 0.f;
-                for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
+                if (lane_dofs) {
+                    gn = wave_dsum(D, lane, [&](int d) { return LF(grad, d) * LF(grad, d); });
+                } else {
+                    for (int d = 0; d < D; d++) gn = gn + LF(grad, d) * LF(grad, d);
+                }
                 gn = sqrtf(gn);
                 improved = gn > tol && (prev - cost) > tol;
                 if (improved && !cg) {
-                    for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+                    if (lane_dofs) {
+                        for (int d = lane; d < D; d += kWave) LF(search, d) = -LF(Mgrad, d);
+                        wave_sync();
+                    } else {
+                        for (int d = 0; d < D; d++) LF(search, d) = -LF(Mgrad, d);
+                    }
                 } else if (improved) {  // solver.py:2865-2884 Polak-Ribiere direction
                     float beta = 0.f, pg_dot_pMg = 0.f;
                     for (int d = 0; d < D; d++) {
