@@ -319,6 +319,49 @@ __device__ __forceinline__ int wave_narrowphase(const DModel& m, const LState& l
     return min(gjk_passes, 0x7fff) << 16 | min(gjk_singles, 0xffff);
 }
 
+// dev_integrate (rigid_solver.py kernel_step_2's semi-implicit Euler) with its dofs, links and coordinates split over
+// the lanes: every entry is computed by the same arithmetic as there, so the result is the same to the bit; the NaN
+// check is a ballot over the lanes.
+__device__ inline void wave_integrate(const DModel& m, const LState& loc, int lane) {
+    const float dt = m.dt;
+    for (int d = lane; d < m.D; d += kWave) FP(loc, vel_next)[d] = FP(loc, vel)[d] + FP(loc, acc)[d] * dt;
+    wave_sync();
+    for (int l = lane; l < m.L; l += kWave) {
+        const int ds = m.link_dof_start[l];
+        if (m.link_dof_end[l] == ds) continue;
+        const int qs = m.link_q_start[l], qe = m.link_q_end[l];
+        const int jt = m.joint_type[m.link_joint_start[l]];
+        const float* q = FP(loc, qpos);
+        const float* v = FP(loc, vel_next);
+        float* qn = FP(loc, qpos_next);
+        if (jt == JT_FREE)
+            for (int k = 0; k < 3; k++) qn[qs + k] = q[qs + k] + v[ds + k] * dt;
+        if (jt == JT_SPHERICAL || jt == JT_FREE) {
+            const int ro = jt == JT_FREE ? 3 : 0;
+            f4q r0{q[qs + ro], q[qs + ro + 1], q[qs + ro + 2], q[qs + ro + 3]};
+            f3 ang = mk3(v[ds + ro], v[ds + ro + 1], v[ds + ro + 2]) * dt;
+            f4q r = qtrans(rotvec_quat(ang, m.eps), r0);
+            qn[qs + ro] = r.w;
+            qn[qs + ro + 1] = r.x;
+            qn[qs + ro + 2] = r.y;
+            qn[qs + ro + 3] = r.z;
+        } else {
+            for (int k = 0; k < qe - qs; k++) qn[qs + k] = q[qs + k] + v[ds + k] * dt;
+        }
+    }
+    wave_sync();
+    bool bad = false;
+    for (int d = lane; d < m.D; d += kWave) bad = bad || isnan(FP(loc, vel_next)[d]);
+    for (int i = lane; i < m.Q; i += kWave) bad = bad || isnan(FP(loc, qpos_next)[i]);
+    if (wave_ballot(bad) == 0) {
+        for (int d = lane; d < m.D; d += kWave) FP(loc, vel)[d] = FP(loc, vel_next)[d];
+        for (int i = lane; i < m.Q; i += kWave) FP(loc, qpos)[i] = FP(loc, qpos_next)[i];
+    } else {
+        FP(loc, errbits)[0] = FP(loc, errbits)[0] | RS_ERR_INVALID_ACC_NAN;
+    }
+    wave_sync();
+}
+
 // One environment's substep on the calling wavefront (w = wavefront index in the workgroup).
 // kPath: 1 = models in the first row-tile class (at most kTileMaxDofs dofs), 4 = the second (up to kTileWideDofs),
 // 0 = the others, 2 = any (decided per call: the shared launch of several models). Returns the narrowphase's GJK/EPA
@@ -377,8 +420,7 @@ __device__ __forceinline__ int fused_env(const DModel& m, const DState& st, cons
     // kernel_step_2: cacc, semi-implicit Euler, forward kinematics and velocities
     wave_update_acc(m, loc, lane, true);
     if (m.integrator != RS_INTEGRATOR_APPROXIMATE_IMPLICITFAST) dev_implicit_damping(m, loc, 0);
-    dev_integrate(m, loc, 0);
-    wave_sync();
+    wave_integrate(m, loc, lane);
     wave_kinematics(m, loc, lane);
     if (!in_place) stage_local<false>(st, (uint32_t*)base, fa.n_local, b, lane);
     return gjk_work;

@@ -1241,7 +1241,11 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     const int D = m.D;
     const int nc = LF(n_con, 0);
     const bool ws = nc > 0 && LF(is_warmstart, 0);
-    for (int d = 0; d < D; d++) LF(qacc, d) = ws ? LF(qacc_ws, d) : LF(acc_smooth, d);
+    // models on the matrix-core path update their dof vectors lane-parallel (entries lane, lane + 64)
+    const bool lane_dofs = KD == 0 && mfma_model(D);
+    for (int d = lane_dofs ? lane : 0; d < D; d += lane_dofs ? kWave : 1)
+        LF(qacc, d) = ws ? LF(qacc_ws, d) : LF(acc_smooth, d);
+    if (lane_dofs) wave_sync();
     for (int d1 = lane; d1 < D; d1 += kWave) {
         int e = m.dof_entity[d1];
         float ma = 0.f;
@@ -1268,8 +1272,6 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
         else if (D <= 2 * kWave) wave_gradient_lanes(m, loc, lane);
         else wave_gradient(m, loc);
     }
-    // models on the matrix-core path update their dof vectors lane-parallel (entries lane, lane + 64)
-    const bool lane_dofs = KD == 0 && mfma_model(D);
     if (lane_dofs) {
         for (int d = lane; d < D; d += kWave) LF(search, d) = -LF(Mgrad, d);
         wave_sync();
@@ -1348,14 +1350,17 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     }
     LF(improved, 0) = improved;
     LF(n_iter, 0) = n_iter;
-    for (int d = 0; d < D; d++) {
+    bool nan_seen = false;
+    for (int d = lane_dofs ? lane : 0; d < D; d += lane_dofs ? kWave : 1) {
         float qa = LF(qacc, d);
         LF(acc, d) = qa;
         LF(qf_constraint, d) = LF(qfrc_constraint, d);
         LF(force, d) = LF(qf_smooth, d) + LF(qfrc_constraint, d);
         LF(qacc_ws, d) = qa;
-        if (isnan(qa)) LF(errbits, 0) = LF(errbits, 0) | RS_ERR_INVALID_FORCE_NAN;
+        nan_seen = nan_seen || isnan(qa);
     }
+    if (lane_dofs ? wave_ballot(nan_seen) != 0 : nan_seen) LF(errbits, 0) = LF(errbits, 0) | RS_ERR_INVALID_FORCE_NAN;
+    if (lane_dofs) wave_sync();
     LF(is_warmstart, 0) = 1;
     // the constraint forces of the rows (ConstraintState.efc_force), from the work field to the state slab
     const int n_rows = LF(n_con, 0);
