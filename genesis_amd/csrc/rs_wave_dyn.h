@@ -161,6 +161,31 @@ __device__ inline void wave_torque_passive(const DModel& m, const LState& loc, i
 
 // forward_dynamics.py:1186-1285 func_update_acc: lane k < 6 owns component k of (cdd_vel, cdd_ang) and, with cacc,
 // lane 6 + k of (cacc_lin, cacc_ang), walking the links in order
+// dev_solve_mass (M out = vec through the L^T D L factor) with the entities split over the lanes: the mass matrix is
+// block-diagonal over entities, and each entity's block is solved by one lane with the same arithmetic as there, so
+// the result is the same to the bit.
+__device__ inline void wave_solve_mass(const DModel& m, const LState& loc, int lane, const float* vec, float* out) {
+    const int D = m.D;
+    const float* L = FP(loc, mass_L);
+    const float* dinv = FP(loc, mass_D_inv);
+    for (int e = lane; e < m.E; e += kWave) {
+        const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
+        for (int ii = 0; ii < d1 - d0; ii++) {
+            const int i = d1 - ii - 1;
+            float cur = vec[i];
+            for (int j = i + 1; j < d1; j++) cur = cur - L[j * D + i] * out[j];
+            out[i] = cur;
+        }
+        for (int i = d0; i < d1; i++) out[i] = out[i] * dinv[i];
+        for (int i = d0; i < d1; i++) {
+            float cur = out[i];
+            for (int j = d0; j < i; j++) cur = cur - L[i * D + j] * out[j];
+            out[i] = cur;
+        }
+    }
+    wave_sync();
+}
+
 __device__ inline void wave_update_acc(const DModel& m, const LState& loc, int lane, bool cacc) {
     const int nlanes = cacc ? 12 : 6;
     if (lane < nlanes) {
@@ -222,8 +247,9 @@ __device__ inline void wave_rne_and_qacc(const DModel& m, const LState& loc, int
         LF(qf_smooth, d) = f;
     }
     wave_sync();
-    dev_solve_mass(m, loc, 0, FV(loc, force, 0), FV(loc, acc_smooth, 0));
-    for (int d = 0; d < m.D; d++) LF(acc, d) = LF(acc_smooth, d);
+    wave_solve_mass(m, loc, lane, FP(loc, force), FP(loc, acc_smooth));
+    for (int d = lane; d < m.D; d += kWave) LF(acc, d) = LF(acc_smooth, d);
+    wave_sync();
 }
 
 // forward_kinematics.py:337-576 func_COM_links_entity

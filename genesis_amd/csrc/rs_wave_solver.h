@@ -518,12 +518,12 @@ __device__ inline void wave_gradient(const DModel& m, const LState& loc) {
 }
 
 // solver.py:2722-2751 with the conjugate-gradient solver's preconditioner, Mgrad = M^-1 grad (solver.py:2737-2747,
-// func_solve_mass_batch; forward_dynamics.py:813-894), uniform across the wave in the reference's order
-__device__ inline void wave_gradient_cg(const DModel& m, const LState& loc) {
+// func_solve_mass_batch; forward_dynamics.py:813-894): the gradient uniform across the wave, the solve one entity per
+// lane (the same arithmetic per entity)
+__device__ inline void wave_gradient_cg(const DModel& m, const LState& loc, int lane) {
     const int D = m.D;
     for (int d = 0; d < D; d++) LF(grad, d) = LF(Ma, d) - LF(force, d) - LF(qfrc_constraint, d);
-    dev_solve_mass(m, loc, 0, FV(loc, grad, 0), FV(loc, Mgrad, 0));
-    wave_sync();
+    wave_solve_mass(m, loc, lane, FP(loc, grad), FP(loc, Mgrad));
 }
 
 // solver.py:2722-2751 + :1893-1910 for models with 25 to 128 dofs (whose factor comes from the matrix cores),
@@ -1264,7 +1264,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
     int improved = nc > 0;
     const bool cg = m.solver_type == RS_SOLVER_CG;
     if (cg) {
-        wave_gradient_cg(m, loc);
+        wave_gradient_cg(m, loc, lane);
     } else {
         if constexpr (KD > 0) wave_hessian_cholesky_tiled<KD>(m, loc, R, lane, tile);
         else wave_hessian_cholesky(m, loc, R, lane);
@@ -1303,7 +1303,7 @@ __device__ inline void wave_solve(const DModel& m, const LState& loc, const DSta
                 else cost = wave_update_constraint(m, loc, R, lane, prev);
                 LF(cost, 0) = cost;
                 if (cg) {
-                    wave_gradient_cg(m, loc);
+                    wave_gradient_cg(m, loc, lane);
                 } else {
                     bool degenerate;
                     if constexpr (KD > 0) degenerate = wave_cholesky_incremental_r<KD>(m, loc, R, lane, tile);
