@@ -43,10 +43,11 @@ __device__ inline void wave_mass_matrix(const DModel& m, const LState& loc, int 
         st3(FP(loc, f_vel), d, 1, 0, fv);
     }
     wave_sync();
-    for (int t = lane; t < D * D; t += kWave) {
-        const int i = t / D, j = t - D * (t / D);
-        const int e = m.dof_entity[i];
-        if (m.dof_entity[j] != e) continue;
+    // the entries of each entity's diagonal block (the others stay zero)
+    for (int e = 0; e < m.E; e++) {
+      const int d0 = m.entity_dof_start[e], n = m.entity_dof_end[e] - d0;
+      for (int u = lane; u < n * n; u += kWave) {
+        const int i = d0 + u / n, j = d0 + (u - n * (u / n)), t = i * D + j;
         const int a = i >= j ? i : j, c = i >= j ? j : i;  // the lower-triangle entry (a, c) carries the value
         float v = (dot3(ld3(FP(loc, f_ang), a, 1, 0), ld3(FP(loc, cdof_ang), c, 1, 0)) +
                    dot3(ld3(FP(loc, f_vel), a, 1, 0), ld3(FP(loc, cdof_vel), c, 1, 0))) *
@@ -59,6 +60,7 @@ __device__ inline void wave_mass_matrix(const DModel& m, const LState& loc, int 
             }
         }
         LF(mass_mat, t) = v;
+      }
     }
     wave_sync();
 }
@@ -67,6 +69,29 @@ __device__ inline void wave_mass_matrix(const DModel& m, const LState& loc, int 
 // separate lanes; row i's entries are read before any lane rescales them
 __device__ inline void wave_factor_mass(const DModel& m, const LState& loc, int lane) {
     const int D = m.D;
+    if (m.E > kWave / 8) {
+        // many entities (a pile of free bodies): one entity per lane, each eliminated sequentially; every entry gets
+        // the same updates in the same order as below, so the factor is the same to the bit
+        for (int e = lane; e < m.E; e += kWave) {
+            const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e];
+            for (int i = d0; i < d1; i++)
+                for (int j = d0; j <= i; j++) LF(mass_L, i * D + j) = LF(mass_mat, i * D + j);
+            for (int ii = 0; ii < d1 - d0; ii++) {
+                const int i = d1 - ii - 1;
+                const float dinv = 1.f / LF(mass_L, i * D + i);
+                LF(mass_D_inv, i) = dinv;
+                for (int jj = 0; jj < i - d0; jj++) {
+                    const int j = i - jj - 1;
+                    const float a = LF(mass_L, i * D + j) * dinv;
+                    for (int k = d0; k <= j; k++) LF(mass_L, j * D + k) = LF(mass_L, j * D + k) - a * LF(mass_L, i * D + k);
+                    LF(mass_L, i * D + j) = a;
+                }
+                LF(mass_L, i * D + i) = 1.f;
+            }
+        }
+        wave_sync();
+        return;
+    }
     for (int e = 0; e < m.E; e++) {
         const int d0 = m.entity_dof_start[e], d1 = m.entity_dof_end[e], n = d1 - d0;
         for (int t = lane; t < n * n; t += kWave) {
