@@ -13,12 +13,16 @@ first, the latest admitted first among equals), but only when evicting them make
 slab is copied whole into pinned host memory and its device memory released; it waits again, and on re-admission
 its slab is restored from that copy and continues from the substep it reached, bit for bit. ``save_request`` writes
 a request (its description, progress and, once started, its state) to a file from which ``load_request`` rebuilds
-it, so a request can also move between servers or processes.
+it, so a request can also move between servers or processes. ``run(..., stop_after_ticks=N, checkpoint_dir=d)``
+stops a run after N shared launches and writes every unfinished request (active, preempted or still waiting) to d;
+``load_checkpoint(d)`` returns them, and a later ``run`` of any server finishes them bit for bit as if uninterrupted.
 """
 
 from __future__ import annotations
 
 import ctypes
+import json
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -209,7 +213,29 @@ class RolloutServer:
         return out if (n < self.max_active and used + head.nbytes <= self.budget) else None
 
     # ------------------------------------------------------------------ loop
-    def run(self, requests: list[Request], keep_final: bool = False) -> dict:
+    def save_checkpoint(self, directory, requests: list[Request]) -> None:
+        """Every request of ``requests`` to ``directory`` (one ``save_request`` file each, and an index)."""
+        os.makedirs(directory, exist_ok=True)
+        names = []
+        for r in requests:
+            name = f"request_{r.rid}.npz"
+            self.save_request(r, os.path.join(directory, name))
+            names.append(name)
+        with open(os.path.join(directory, "index.json"), "w") as f:
+            json.dump({"requests": names}, f)
+
+    def load_checkpoint(self, directory) -> list[Request]:
+        """The requests a ``save_checkpoint`` wrote, ready for ``run`` (started ones resume from their state)."""
+        with open(os.path.join(directory, "index.json")) as f:
+            names = json.load(f)["requests"]
+        return [self.load_request(os.path.join(directory, n)) for n in names]
+
+    def run(self, requests: list[Request], keep_final: bool = False, stop_after_ticks: int | None = None,
+            checkpoint_dir=None) -> dict:
+        """Serve ``requests`` until all are finished, or until ``stop_after_ticks`` shared launches when it is given:
+        then the unfinished requests are written to ``checkpoint_dir`` (``save_checkpoint``) and released."""
+        if stop_after_ticks is not None and checkpoint_dir is None:
+            raise ValueError("stop_after_ticks needs a checkpoint_dir")
         waiting = sorted(requests, key=lambda r: r.arrival)
         start = {id(r): r.steps_done for r in waiting}  # requests loaded from a checkpoint resume part-way
         for r in waiting:
@@ -220,9 +246,18 @@ class RolloutServer:
                 raise ValueError(f"request {r.rid} is finished or already resident")
         active: list[Request] = []
         done: list[Request] = []
+        ticks = 0
+        stopped = False
         t0 = time.perf_counter()
         with torch.cuda.stream(self.stream):
             while waiting or active:
+                if stop_after_ticks is not None and ticks >= stop_after_ticks:
+                    self.save_checkpoint(checkpoint_dir, active + waiting)
+                    for r in active:
+                        r.slab = None
+                        self.used -= r.nbytes
+                    stopped = True
+                    break
                 now = time.perf_counter() - t0
                 while True:
                     # the arrived request of highest priority, the earliest among equals
@@ -248,6 +283,7 @@ class RolloutServer:
                     continue
                 self.tick_ms.append(self.tick(active))
                 self.active_per_tick.append(len(active))
+                ticks += 1
                 still = []
                 for r in active:
                     if r.steps_done >= r.horizon:
@@ -271,10 +307,11 @@ class RolloutServer:
             "env_steps_per_s": env_steps / wall,
             "env_steps_per_busy_s": env_steps / busy if busy > 0 else None,
             "ticks": len(self.tick_ms),
-            "p50_step_ms": float(np.percentile(self.tick_ms, 50)),
-            "p99_step_ms": float(np.percentile(self.tick_ms, 99)),
-            "mean_active": float(np.mean(self.active_per_tick)),
-            "max_active": int(np.max(self.active_per_tick)),
-            "mean_queue_s": float(np.mean([r.admitted - r.arrival for r in done])),
+            "p50_step_ms": float(np.percentile(self.tick_ms, 50)) if self.tick_ms else None,
+            "p99_step_ms": float(np.percentile(self.tick_ms, 99)) if self.tick_ms else None,
+            "mean_active": float(np.mean(self.active_per_tick)) if self.active_per_tick else 0.0,
+            "max_active": int(np.max(self.active_per_tick)) if self.active_per_tick else 0,
+            "mean_queue_s": float(np.mean([r.admitted - r.arrival for r in done])) if done else None,
             "preemptions": self.preemptions,
+            "stopped": stopped,
         }

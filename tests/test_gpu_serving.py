@@ -147,3 +147,29 @@ def test_request_eviction_and_file_checkpoint_round_trip(tmp_path):
         np.savez(f, **arrays)
     with pytest.raises(ValueError):
         server.load_request(p3)
+
+
+def test_stopped_run_resumes_from_its_checkpoint_bit_exact(tmp_path):
+    """A run stopped after 25 shared launches writes its unfinished requests (active ones with their state, one not
+    yet admitted without) to a checkpoint; a fresh server loads it and finishes them: every request ends bit-equal to
+    its run alone."""
+    import os
+
+    ck = os.path.join(str(tmp_path), "ck")
+    reqs = [Request(0, "franka", 64, 80, 0.0), Request(1, "quadruped", 96, 60, 0.0), Request(2, "boxpile", 32, 40, 0.0),
+            Request(3, "franka", 48, 30, 30.0)]  # arrives long after the stop
+    first = RolloutServer(hbm_budget_bytes=64 << 30, max_active=8)
+    stats = first.run(reqs, stop_after_ticks=25, checkpoint_dir=ck)
+    assert stats["stopped"] and stats["requests"] == 0 and first.used == 0
+    second = RolloutServer(hbm_budget_bytes=64 << 30, max_active=8)
+    back = second.load_checkpoint(ck)
+    assert sorted((r.rid, r.steps_done, r.host_state is not None) for r in back) == [
+        (0, 25, True), (1, 25, True), (2, 25, True), (3, 0, False)]
+    for r in back:
+        r.arrival = 0.0
+    print("[serving] resuming 4 requests from the checkpoint", flush=True)
+    stats = second.run(back, keep_final=True)
+    assert stats["requests"] == 4 and not stats["stopped"] and second.used == 0
+    for r in back:
+        q, v = _alone_like(second, r)
+        assert torch.equal(q, r.final_qpos) and torch.equal(v, r.final_vel), f"request {r.rid} differs from alone"
